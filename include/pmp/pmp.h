@@ -1,0 +1,207 @@
+/*
+ * pmp.h -- C-ABI of the MI355X-native PatchMatch-Stereo-Panorama densifier.
+ *
+ * Plain C linkage, POD arguments, no exceptions across the boundary.  Every
+ * entry point returns PMP_OK (0) or a negative PMP_E* status; the reason is
+ * available from pmp_last_error().  One context per host thread (contexts
+ * are not internally locked); a context owns its device memory, its HIP
+ * stream and its image pool.  The caller owns all host buffers.
+ *
+ * Each entry point names the reference interface it replaces (paths are in
+ * the RoblabWh/PatchMatch tree).  The reference's CUDA boundary is the set
+ * of extern "C" declarations at src/openvslam/depthmap.cc:11-21, defined in
+ * src/openvslam/PatchMatchCuda.cu:419-912, driven by
+ * DepthmapEstimator::ComputePatchMatch (depthmap.cc:541-806); its C++
+ * boundary is dense::DepthmapEstimator / DepthmapCleaner / DepthmapPruner
+ * (depthmap.h:122-291).
+ *
+ * Conventions:
+ *   images   uint8 grey, equirectangular, row pitch in bytes;
+ *   poses    world->camera rotation (row-major) and translation, doubles
+ *            (data/keyframe.cc:235-243: rot_cw, trans_cw);
+ *   planes   p with p.X = 1 for camera-frame points X on the plane, so the
+ *            depth along the unit bearing b is 1/|b.p| (PatchMatchCuda.cu:31-43);
+ *   maps     H*W, row-major; plane maps as 4 floats per pixel (x, y, z, 0).
+ */
+#ifndef PMP_PMP_H
+#define PMP_PMP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PMP_MAX_VIEWS 10
+
+enum {
+    PMP_OK = 0,
+    PMP_EINVAL = -1,     /* bad argument */
+    PMP_ENOMEM = -2,     /* device or host allocation failed */
+    PMP_EDEVICE = -3,    /* HIP runtime error */
+    PMP_ENOTFOUND = -4,  /* unknown image id */
+    PMP_ESTATE = -5      /* call out of order (e.g. no views set) */
+};
+
+enum { PMP_AGG_REF_FOLD = 0, PMP_AGG_MAX = 1, PMP_AGG_TOPK_MEAN = 2 };
+enum { PMP_EDGE_WRAP = 0, PMP_EDGE_CLAMP = 1, PMP_EDGE_ZERO = 2 };
+enum { PMP_COVER_REF_GRID = 0, PMP_COVER_FULL = 1 };
+enum { PMP_INIT_REF_UNIFORM = 0, PMP_INIT_INVDEPTH_HEMI = 1 };
+enum { PMP_MAPS_WORK = 0, PMP_MAPS_INIT = 1, PMP_MAPS_FINAL = 2 };
+
+typedef struct pmp_ctx pmp_ctx;
+
+/* Every constant the reference hard-codes, with the reference value as the
+ * default (pmp_default_params).  Layout is part of the ABI. */
+typedef struct pmp_params {
+    int32_t patch_half;        /* 5: patch_size_-2 (depthmap.cc:666,725), taps -half..half */
+    int32_t patch_stride;      /* 2: PatchMatchCuda.cu:104-106 */
+    int32_t iterations;        /* red-black iterations (Dense.Iterations) */
+    int32_t n_refine;          /* 6 random refinements per pass (:363-395), <= 8 */
+    int32_t agg_mode;          /* PMP_AGG_*: view aggregation (:170-171 fold) */
+    int32_t topk;              /* k of PMP_AGG_TOPK_MEAN */
+    int32_t edge_mode;         /* PMP_EDGE_*: neighbour sampling outside the image */
+    int32_t coverage_mode;     /* PMP_COVER_*: reference launch grids or every interior pixel */
+    int32_t init_mode;         /* PMP_INIT_*: random hypothesis distribution */
+    int32_t postprocess;       /* 1: run PostProcess (depthmap.cc:1169-1208) */
+    uint64_t seed;             /* key of the Philox streams */
+    float min_depth, max_depth;            /* init range and PostProcess range */
+    float init_inv_depth_min, init_inv_depth_max; /* PMP_INIT_INVDEPTH_HEMI range (1/m) */
+    float min_score;           /* 0.1: PostProcess score threshold */
+    float sigma_color;         /* 50 */
+    float sigma_space;         /* 5 */
+    float var_floor;           /* 0.1: NCC variance floor */
+    float fold_agree;          /* 0.25: fold agreement */
+    float depth_penalty;       /* 1250 */
+    float plane_penalty;       /* 8 */
+    float refine_depth_range;  /* 0.02 */
+    float refine_normal_range; /* 0.5 */
+    float refine_depth_decay;  /* 0.3 */
+    float refine_normal_decay; /* 0.8 */
+    float min_patch_sd;        /* 0: low-texture gate (depthmap.cc:739) */
+    float median_rel_thr;      /* 0.05: PostProcess median test */
+} pmp_params;
+
+typedef struct pmp_pose {
+    double R[9];   /* world->camera rotation, row-major */
+    double t[3];   /* world->camera translation */
+} pmp_pose;
+
+/* ---- context ----------------------------------------------------------- */
+
+/* Replaces the per-call cudaMalloc/cudaFree of DepthmapEstimator
+ * (depthmap.cc:522,557-558,652-656,771-800): buffers live in the context and
+ * grow on demand.  device: HIP device ordinal. */
+int pmp_create(pmp_ctx** out, int32_t device);
+void pmp_destroy(pmp_ctx* ctx);
+/* Replaces printing cudaGetErrorString and continuing (PatchMatchCuda.cu:447-451). */
+const char* pmp_last_error(const pmp_ctx* ctx);
+/* Reference operating values. */
+void pmp_default_params(pmp_params* p);
+/* Waits for the context's stream. */
+int pmp_synchronize(pmp_ctx* ctx);
+
+/* ---- images ------------------------------------------------------------ */
+
+/* Upload (or replace) a grey image in the context's pool under id.
+ * Replaces InitTextureImages (PatchMatchCuda.cu:697-797, float textures built
+ * from depthmap.cc:569-571); images stay resident as uint8 plus a packed
+ * 2x2-neighbourhood copy for bilinear sampling.  on_device != 0: pixels is a
+ * device pointer on the context's device. */
+int pmp_image_put(pmp_ctx* ctx, uint64_t id, const uint8_t* pixels, int32_t width, int32_t height,
+                  int64_t pitch, int32_t on_device);
+int pmp_image_drop(pmp_ctx* ctx, uint64_t id);
+/* Number of images and device bytes held by the pool. */
+int pmp_image_pool_stats(const pmp_ctx* ctx, int64_t* count, int64_t* bytes);
+
+/* ---- one keyframe (DepthmapEstimator, depthmap.cc:541-806) -------------- */
+
+/* Choose the reference image and its neighbours for the next calls.
+ * Replaces InitConstantMem (PatchMatchCuda.cu:885-912) and the pose algebra
+ * of depthmap.cc:577-649 (ref pose = rot_cw/trans_cw of the centre keyframe,
+ * neighbour poses = their own rot_cw/trans_cw). */
+int pmp_set_views(pmp_ctx* ctx, uint64_t ref_id, const pmp_pose* ref_pose, int32_t n_views,
+                  const uint64_t* view_ids, const pmp_pose* view_poses);
+
+/* RandomInitialization (PatchMatchCuda.cu:656-663): the memsets of
+ * depthmap.cc:657-659, random depth/plane per pixel and the initial score.
+ * Also snapshots the maps for pmp_finalize (the reference keeps them in the
+ * host result, depthmap.cc:673-687).  Replaces ComputeRandomSeed too: the
+ * random streams are counter-based, keyed on params->seed. */
+int pmp_random_initialization(pmp_ctx* ctx, const pmp_params* params);
+
+/* ScorePlaneDepth (PatchMatchCuda.cu:528-565): score the working plane map
+ * at every pixel of the scoring grid (params->coverage_mode). */
+int pmp_score_plane_depth(pmp_ctx* ctx, const pmp_params* params);
+
+/* PatchMatchRedBlackPass (PatchMatchCuda.cu:568-615): iterations
+ * [it_begin, it_end), each a pass over (i+j) even pixels then (i+j) odd. */
+int pmp_red_black_pass(pmp_ctx* ctx, const pmp_params* params, int32_t it_begin, int32_t it_end);
+
+/* Copy-back window, low-texture gate and PostProcess of ComputePatchMatch
+ * (depthmap.cc:673-687, :731-767, :1169-1208).  mask: host H*W bytes,
+ * 0 = ignored pixel; NULL = all valid. */
+int pmp_finalize(pmp_ctx* ctx, const pmp_params* params, const uint8_t* mask);
+
+/* The whole estimator: init, params->iterations red-black iterations,
+ * finalize.  Replaces the device part of DepthmapEstimator::ComputePatchMatch
+ * (depthmap.cc:555-767). */
+int pmp_densify(pmp_ctx* ctx, const pmp_params* params, const uint8_t* mask);
+
+/* Upload working maps (replaces the H->D copies of depthmap.cc:717-719).
+ * plane4: H*W*4 floats; any pointer may be NULL (left unchanged). */
+int pmp_set_maps(pmp_ctx* ctx, const float* plane4, const float* depth, const float* score);
+
+/* Download maps (replaces the D->H copies of depthmap.cc:669-671, :727-729).
+ * which: PMP_MAPS_WORK (working maps), PMP_MAPS_INIT (after initialisation)
+ * or PMP_MAPS_FINAL (DepthmapEstimatorResult: depth, plane, score, nghbr,
+ * depthmap.h:71-79).  plane gets 4 floats per pixel; NULL pointers skipped;
+ * nghbr only for PMP_MAPS_FINAL. */
+int pmp_get_maps(pmp_ctx* ctx, int32_t which, float* plane4, float* depth, float* score, int32_t* nghbr);
+
+/* Counters of the last pmp_red_black_pass/pmp_densify: hypotheses scored
+ * (evals) and scored+pre-rejected (candidates), summed over pixels. */
+int pmp_eval_counts(pmp_ctx* ctx, int64_t* evals, int64_t* candidates);
+
+/* ---- geometric consistency (DepthmapCleaner, depthmap.cc:1320-1377) ----- */
+
+/* depth_maps[0] is the map to clean (the reference swaps its centre map to
+ * index 0), depth_maps[1..n_maps-1] the others; host pointers, W*H floats.
+ * A pixel is kept when at least min_consistent maps (itself included) agree
+ * within same_depth_thr * depth. */
+int pmp_clean(pmp_ctx* ctx, int32_t width, int32_t height, int32_t n_maps, const float* const* depth_maps,
+              const pmp_pose* poses, float same_depth_thr, int32_t min_consistent, float* cleaned);
+
+/* ---- point fusion (DepthmapPruner, depthmap.cc:1405-1696) --------------- */
+
+/* AddView (depthmap.cc:1405-1423): append a cleaned depth map, its pose and
+ * its BGR thumbnail (W*H*3 bytes) to the device-resident queue. */
+int pmp_pruner_add_view(pmp_ctx* ctx, const float* cleaned_depth, const pmp_pose* pose, const uint8_t* bgr,
+                        int32_t width, int32_t height);
+/* Prune (depthmap.cc:1535-1696): fuse the head map against the later maps
+ * (which are modified, as in the reference); results are identical to the
+ * reference's raster-order loop.  Writes the pruned head map (nullable) and
+ * up to cap points: pts (3 doubles, world frame), cols (3 bytes BGR), kps
+ * (row, col); *count gets the number of kept points (may exceed cap). */
+int pmp_pruner_prune(pmp_ctx* ctx, float same_depth_thr, float* pruned_depth, int64_t cap, double* pts,
+                     uint8_t* cols, int32_t* kps, int64_t* count);
+int pmp_pruner_pop_head(pmp_ctx* ctx);           /* PopHead (depthmap.cc:1429-1446) */
+int pmp_pruner_reset(pmp_ctx* ctx);              /* reset (depthmap.cc:1478-1499) */
+int pmp_pruner_size(const pmp_ctx* ctx);         /* queue length */
+int pmp_pruner_get_map(pmp_ctx* ctx, int32_t index, float* depth);  /* read back a queued map */
+
+/* ---- diagnostics --------------------------------------------------------- */
+
+/* Evaluate the device elementary functions (see pmp_math.h) on n inputs:
+ * which 0 rcp 1 rsqrt 2 sqrt 3 asin 4 acos 5 atan2(a,b) 6 exp 7 log 8 sin2pi
+ * 9 cos2pi.  Host pointers. */
+int pmp_debug_eval_math(pmp_ctx* ctx, int32_t which, int64_t n, const float* a, const float* b, float* out);
+/* The standard normals of refinement for pixel (i, j), iteration it (W wide
+ * image, params->seed), n <= 24. */
+int pmp_debug_refine_normals(pmp_ctx* ctx, const pmp_params* params, int32_t width, int32_t i, int32_t j,
+                             int32_t it, int32_t n, float* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

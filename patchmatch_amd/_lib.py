@@ -1,0 +1,274 @@
+"""ctypes binding of libpmp.so (the C-ABI of include/pmp/pmp.h).
+
+There is no fallback: if the HIP library is not built, or no GPU is
+visible, every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from .params import Params
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpmp.so")
+
+PMP_MAPS_WORK, PMP_MAPS_INIT, PMP_MAPS_FINAL = 0, 1, 2
+STATUS = {0: "PMP_OK", -1: "PMP_EINVAL", -2: "PMP_ENOMEM", -3: "PMP_EDEVICE", -4: "PMP_ENOTFOUND", -5: "PMP_ESTATE"}
+
+_f32p = ctypes.POINTER(ctypes.c_float)
+_f64p = ctypes.POINTER(ctypes.c_double)
+_u8p = ctypes.POINTER(ctypes.c_uint8)
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_i64p = ctypes.POINTER(ctypes.c_int64)
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+_vp = ctypes.c_void_p
+
+
+class Pose(ctypes.Structure):
+    _fields_ = [("R", ctypes.c_double * 9), ("t", ctypes.c_double * 3)]
+
+    @classmethod
+    def from12(cls, v):
+        v = np.asarray(v, dtype=np.float64).reshape(12)
+        return cls((ctypes.c_double * 9)(*v[:9]), (ctypes.c_double * 3)(*v[9:]))
+
+
+EXPORTS = [
+    "pmp_create", "pmp_destroy", "pmp_last_error", "pmp_default_params", "pmp_synchronize",
+    "pmp_image_put", "pmp_image_drop", "pmp_image_pool_stats", "pmp_set_views",
+    "pmp_random_initialization", "pmp_score_plane_depth", "pmp_red_black_pass", "pmp_finalize",
+    "pmp_densify", "pmp_set_maps", "pmp_get_maps", "pmp_eval_counts", "pmp_clean",
+    "pmp_pruner_add_view", "pmp_pruner_prune", "pmp_pruner_pop_head", "pmp_pruner_reset",
+    "pmp_pruner_size", "pmp_pruner_get_map", "pmp_debug_eval_math", "pmp_debug_refine_normals",
+]
+
+_lib = None
+
+
+class PmpError(RuntimeError):
+    pass
+
+
+def load():
+    """Load libpmp.so and declare its signatures (raises if it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise PmpError(f"{LIB_PATH} is not built; run __graft_entry__.build() (no CPU fallback exists)")
+    L = ctypes.CDLL(LIB_PATH)
+    P = ctypes.POINTER
+    c_int, i32, i64, u64, f32 = ctypes.c_int, ctypes.c_int32, ctypes.c_int64, ctypes.c_uint64, ctypes.c_float
+    sig = {
+        "pmp_create": ([P(_vp), i32], c_int),
+        "pmp_destroy": ([_vp], None),
+        "pmp_last_error": ([_vp], ctypes.c_char_p),
+        "pmp_default_params": ([P(Params)], None),
+        "pmp_synchronize": ([_vp], c_int),
+        "pmp_image_put": ([_vp, u64, _u8p, i32, i32, i64, i32], c_int),
+        "pmp_image_drop": ([_vp, u64], c_int),
+        "pmp_image_pool_stats": ([_vp, _i64p, _i64p], c_int),
+        "pmp_set_views": ([_vp, u64, P(Pose), i32, _u64p, P(Pose)], c_int),
+        "pmp_random_initialization": ([_vp, P(Params)], c_int),
+        "pmp_score_plane_depth": ([_vp, P(Params)], c_int),
+        "pmp_red_black_pass": ([_vp, P(Params), i32, i32], c_int),
+        "pmp_finalize": ([_vp, P(Params), _u8p], c_int),
+        "pmp_densify": ([_vp, P(Params), _u8p], c_int),
+        "pmp_set_maps": ([_vp, _f32p, _f32p, _f32p], c_int),
+        "pmp_get_maps": ([_vp, i32, _f32p, _f32p, _f32p, _i32p], c_int),
+        "pmp_eval_counts": ([_vp, _i64p, _i64p], c_int),
+        "pmp_clean": ([_vp, i32, i32, i32, P(_f32p), P(Pose), f32, i32, _f32p], c_int),
+        "pmp_pruner_add_view": ([_vp, _f32p, P(Pose), _u8p, i32, i32], c_int),
+        "pmp_pruner_prune": ([_vp, f32, _f32p, i64, _f64p, _u8p, _i32p, _i64p], c_int),
+        "pmp_pruner_pop_head": ([_vp], c_int),
+        "pmp_pruner_reset": ([_vp], c_int),
+        "pmp_pruner_size": ([_vp], c_int),
+        "pmp_pruner_get_map": ([_vp, i32, _f32p], c_int),
+        "pmp_debug_eval_math": ([_vp, i32, i64, _f32p, _f32p, _f32p], c_int),
+        "pmp_debug_refine_normals": ([_vp, P(Params), i32, i32, i32, i32, i32, _f32p], c_int),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = L
+    return L
+
+
+def _p(a, t):
+    return None if a is None else a.ctypes.data_as(t)
+
+
+class Context:
+    """One pmp_ctx: device memory, a HIP stream and an image pool."""
+
+    def __init__(self, device: int = 0):
+        self.L = load()
+        h = _vp()
+        rc = self.L.pmp_create(ctypes.byref(h), device)
+        if rc != 0:
+            raise PmpError(f"pmp_create(device={device}) failed: {STATUS.get(rc, rc)} (is a GPU visible?)")
+        self.h = h
+        self.W = self.H = 0
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.pmp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _chk(self, rc, what):
+        if rc != 0:
+            msg = self.L.pmp_last_error(self.h).decode(errors="replace")
+            raise PmpError(f"{what}: {STATUS.get(rc, rc)}: {msg}")
+
+    # -- images / views
+    def image_put(self, image_id: int, img):
+        img = np.ascontiguousarray(img, dtype=np.uint8)
+        H, W = img.shape
+        self._chk(self.L.pmp_image_put(self.h, image_id, _p(img, _u8p), W, H, W, 0), "pmp_image_put")
+
+    def image_put_device(self, image_id: int, ptr: int, W: int, H: int, pitch: int):
+        self._chk(self.L.pmp_image_put(self.h, image_id, ctypes.cast(ptr, _u8p), W, H, pitch, 1), "pmp_image_put")
+
+    def image_drop(self, image_id: int):
+        self._chk(self.L.pmp_image_drop(self.h, image_id), "pmp_image_drop")
+
+    def pool_stats(self):
+        c, b = ctypes.c_int64(), ctypes.c_int64()
+        self._chk(self.L.pmp_image_pool_stats(self.h, ctypes.byref(c), ctypes.byref(b)), "pmp_image_pool_stats")
+        return c.value, b.value
+
+    def set_views(self, ref_id, ref_pose, view_ids, view_poses):
+        n = len(view_ids)
+        ids = (ctypes.c_uint64 * n)(*view_ids)
+        poses = (Pose * n)(*[Pose.from12(p) for p in view_poses])
+        self._chk(self.L.pmp_set_views(self.h, ref_id, ctypes.byref(Pose.from12(ref_pose)), n, ids, poses),
+                  "pmp_set_views")
+
+    def load_keyframe(self, ref, views, ref_pose, view_poses, base_id=0):
+        """Upload reference + neighbours (ids base_id, base_id+1, ...) and select them."""
+        self.image_put(base_id, ref)
+        for k, v in enumerate(views):
+            self.image_put(base_id + 1 + k, v)
+        self.set_views(base_id, ref_pose, [base_id + 1 + k for k in range(len(views))], view_poses)
+        self.H, self.W = np.asarray(ref).shape
+
+    # -- stages
+    def random_initialization(self, p: Params):
+        self._chk(self.L.pmp_random_initialization(self.h, ctypes.byref(p)), "pmp_random_initialization")
+
+    def score_plane_depth(self, p: Params):
+        self._chk(self.L.pmp_score_plane_depth(self.h, ctypes.byref(p)), "pmp_score_plane_depth")
+
+    def red_black_pass(self, p: Params, it_begin: int, it_end: int):
+        self._chk(self.L.pmp_red_black_pass(self.h, ctypes.byref(p), it_begin, it_end), "pmp_red_black_pass")
+
+    def finalize(self, p: Params, mask=None):
+        m = None if mask is None else np.ascontiguousarray(mask, dtype=np.uint8)
+        self._chk(self.L.pmp_finalize(self.h, ctypes.byref(p), _p(m, _u8p)), "pmp_finalize")
+
+    def densify(self, p: Params, mask=None):
+        m = None if mask is None else np.ascontiguousarray(mask, dtype=np.uint8)
+        self._chk(self.L.pmp_densify(self.h, ctypes.byref(p), _p(m, _u8p)), "pmp_densify")
+
+    def synchronize(self):
+        self._chk(self.L.pmp_synchronize(self.h), "pmp_synchronize")
+
+    def set_maps(self, plane4=None, depth=None, score=None):
+        a = None if plane4 is None else np.ascontiguousarray(plane4, dtype=np.float32)
+        b = None if depth is None else np.ascontiguousarray(depth, dtype=np.float32)
+        c = None if score is None else np.ascontiguousarray(score, dtype=np.float32)
+        self._chk(self.L.pmp_set_maps(self.h, _p(a, _f32p), _p(b, _f32p), _p(c, _f32p)), "pmp_set_maps")
+
+    def get_maps(self, which=PMP_MAPS_WORK):
+        H, W = self.H, self.W
+        plane = np.zeros((H, W, 4), np.float32)
+        depth = np.zeros((H, W), np.float32)
+        score = np.zeros((H, W), np.float32)
+        nghbr = np.zeros((H, W), np.int32) if which == PMP_MAPS_FINAL else None
+        self._chk(self.L.pmp_get_maps(self.h, which, _p(plane, _f32p), _p(depth, _f32p), _p(score, _f32p),
+                                      _p(nghbr, _i32p)), "pmp_get_maps")
+        return (plane, depth, score, nghbr) if which == PMP_MAPS_FINAL else (plane, depth, score)
+
+    def eval_counts(self):
+        e, c = ctypes.c_int64(), ctypes.c_int64()
+        self._chk(self.L.pmp_eval_counts(self.h, ctypes.byref(e), ctypes.byref(c)), "pmp_eval_counts")
+        return e.value, c.value
+
+    # -- cleaner / pruner
+    def clean(self, depths, poses, thr, min_consistent):
+        depths = [np.ascontiguousarray(d, dtype=np.float32) for d in depths]
+        H, W = depths[0].shape
+        arr = (_f32p * len(depths))(*[_p(d, _f32p) for d in depths])
+        ps = (Pose * len(depths))(*[Pose.from12(p) for p in poses])
+        out = np.zeros((H, W), np.float32)
+        self._chk(self.L.pmp_clean(self.h, W, H, len(depths), arr, ps, thr, min_consistent, _p(out, _f32p)),
+                  "pmp_clean")
+        return out
+
+    def pruner_add_view(self, depth, pose, bgr):
+        d = np.ascontiguousarray(depth, dtype=np.float32)
+        b = np.ascontiguousarray(bgr, dtype=np.uint8)
+        H, W = d.shape
+        self._chk(self.L.pmp_pruner_add_view(self.h, _p(d, _f32p), ctypes.byref(Pose.from12(pose)), _p(b, _u8p), W, H),
+                  "pmp_pruner_add_view")
+
+    def pruner_prune(self, thr, shape):
+        H, W = shape
+        cap = H * W
+        pruned = np.zeros((H, W), np.float32)
+        pts = np.zeros((cap, 3), np.float64)
+        cols = np.zeros((cap, 3), np.uint8)
+        kps = np.zeros((cap, 2), np.int32)
+        cnt = ctypes.c_int64()
+        self._chk(self.L.pmp_pruner_prune(self.h, thr, _p(pruned, _f32p), cap, _p(pts, _f64p), _p(cols, _u8p),
+                                          _p(kps, _i32p), ctypes.byref(cnt)), "pmp_pruner_prune")
+        n = cnt.value
+        return pruned, pts[:n], cols[:n], kps[:n]
+
+    def pruner_pop_head(self):
+        self._chk(self.L.pmp_pruner_pop_head(self.h), "pmp_pruner_pop_head")
+
+    def pruner_reset(self):
+        self._chk(self.L.pmp_pruner_reset(self.h), "pmp_pruner_reset")
+
+    def pruner_size(self):
+        return self.L.pmp_pruner_size(self.h)
+
+    def pruner_get_map(self, index, shape):
+        out = np.zeros(shape, np.float32)
+        self._chk(self.L.pmp_pruner_get_map(self.h, index, _p(out, _f32p)), "pmp_pruner_get_map")
+        return out
+
+    # -- diagnostics
+    MATH = {"rcp": 0, "rsqrt": 1, "sqrt": 2, "asin": 3, "acos": 4, "atan2": 5, "exp": 6, "log": 7,
+            "sin2pi": 8, "cos2pi": 9}
+
+    def eval_math(self, name, a, b=None):
+        a = np.ascontiguousarray(a, dtype=np.float32)
+        bb = None if b is None else np.ascontiguousarray(b, dtype=np.float32)
+        out = np.zeros_like(a)
+        self._chk(self.L.pmp_debug_eval_math(self.h, self.MATH[name], a.size, _p(a, _f32p), _p(bb, _f32p),
+                                             _p(out, _f32p)), "pmp_debug_eval_math")
+        return out
+
+    def refine_normals(self, p: Params, W, i, j, it, n=18):
+        out = np.zeros(n, np.float32)
+        self._chk(self.L.pmp_debug_refine_normals(self.h, ctypes.byref(p), W, i, j, it, n, _p(out, _f32p)),
+                  "pmp_debug_refine_normals")
+        return out

@@ -1,0 +1,265 @@
+// pmp_context.hip -- context, errors, image pool, workspace and host tables.
+#include <math.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "pmp_internal.h"
+
+namespace pmp {
+
+int set_error(pmp_ctx* ctx, int code, const std::string& msg) {
+    if (ctx) ctx->last_error = msg;
+    return code;
+}
+
+int check_hip(pmp_ctx* ctx, hipError_t e, const char* what) {
+    std::string m = std::string(what) + ": " + hipGetErrorString(e);
+    return set_error(ctx, e == hipErrorOutOfMemory ? PMP_ENOMEM : PMP_EDEVICE, m);
+}
+
+// Bearing tables (PatchMatchCuda.cu:191-198), double precision rounded once:
+// rowtab[i] = (cos lat_i, sin lat_i), lat_i = -pi (i/H - 1/2);
+// coltab[j] = (sin lon_j, cos lon_j), lon_j = 2 pi (j/W - 1/2).
+void host_tables(int W, int H, std::vector<float2>& rowtab, std::vector<float2>& coltab) {
+    rowtab.resize(H);
+    coltab.resize(W);
+    for (int i = 0; i < H; ++i) {
+        double lat = -M_PI * ((double)i / (double)H - 0.5);
+        rowtab[i] = make_float2((float)cos(lat), (float)sin(lat));
+    }
+    for (int j = 0; j < W; ++j) {
+        double lon = 2.0 * M_PI * ((double)j / (double)W - 0.5);
+        coltab[j] = make_float2((float)sin(lon), (float)cos(lon));
+    }
+}
+
+// M = R_n R_ref^T, c = t_n - M t_ref in double, rounded once.  Equivalent to
+// the two transforms of PatchMatchCuda.cu:124-130 with R1_inv = R_ref^T,
+// t1_inv = -R_ref^T t_ref (depthmap.cc:604-628) and the neighbours' own
+// world->camera poses (depthmap.cc:631-649).
+void derive_poses(const double* ref_pose, int N, const double* view_poses, float (*M)[9], float (*c)[3]) {
+    const double* R0 = ref_pose;
+    const double* t0 = ref_pose + 9;
+    for (int n = 0; n < N; ++n) {
+        const double* Rn = view_poses + 12 * n;
+        const double* tn = Rn + 9;
+        double Md[9];
+        for (int r = 0; r < 3; ++r)
+            for (int col = 0; col < 3; ++col) {
+                double s = 0.0;
+                for (int k = 0; k < 3; ++k) s += Rn[3 * r + k] * R0[3 * col + k];
+                Md[3 * r + col] = s;
+            }
+        for (int r = 0; r < 3; ++r) {
+            double s = tn[r];
+            for (int k = 0; k < 3; ++k) s -= Md[3 * r + k] * t0[k];
+            c[n][r] = (float)s;
+        }
+        for (int k = 0; k < 9; ++k) M[n][k] = (float)Md[k];
+    }
+}
+
+template <typename T>
+static int grow(pmp_ctx* ctx, T*& p, size_t n) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    PMP_HIP(ctx, hipMalloc(&p, n * sizeof(T)));
+    return PMP_OK;
+}
+
+int ensure_workspace(pmp_ctx* ctx, int W, int H) {
+    const size_t P = (size_t)W * H;
+    if (P <= ctx->ws_pixels) return PMP_OK;
+    PMP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    int rc = 0;
+    rc |= grow(ctx, ctx->work.plane, P);
+    rc |= grow(ctx, ctx->work.depth, P);
+    rc |= grow(ctx, ctx->work.score, P);
+    rc |= grow(ctx, ctx->init.plane, P);
+    rc |= grow(ctx, ctx->init.depth, P);
+    rc |= grow(ctx, ctx->init.score, P);
+    rc |= grow(ctx, ctx->out_plane, P);
+    rc |= grow(ctx, ctx->out_depth, P);
+    rc |= grow(ctx, ctx->out_score, P);
+    rc |= grow(ctx, ctx->out_nghbr, P);
+    rc |= grow(ctx, ctx->tmp0, P);
+    rc |= grow(ctx, ctx->tmp1, P);
+    rc |= grow(ctx, ctx->mask, P);
+    if (rc) return rc;
+    if (!ctx->counters) PMP_HIP(ctx, hipMalloc(&ctx->counters, 4 * sizeof(unsigned long long)));
+    PMP_HIP(ctx, hipMemsetAsync(ctx->counters, 0, 4 * sizeof(unsigned long long), ctx->stream));
+    ctx->ws_pixels = P;
+    return PMP_OK;
+}
+
+int ensure_tables(pmp_ctx* ctx, int W, int H, float sigma_color) {
+    if (ctx->tab_W != W || ctx->tab_H != H || !ctx->rowtab) {
+        std::vector<float2> rt, ct;
+        host_tables(W, H, rt, ct);
+        if (ctx->rowtab) (void)hipFree(ctx->rowtab);
+        if (ctx->coltab) (void)hipFree(ctx->coltab);
+        ctx->rowtab = nullptr;
+        ctx->coltab = nullptr;
+        PMP_HIP(ctx, hipMalloc(&ctx->rowtab, H * sizeof(float2)));
+        PMP_HIP(ctx, hipMalloc(&ctx->coltab, W * sizeof(float2)));
+        PMP_HIP(ctx, hipMemcpy(ctx->rowtab, rt.data(), H * sizeof(float2), hipMemcpyHostToDevice));
+        PMP_HIP(ctx, hipMemcpy(ctx->coltab, ct.data(), W * sizeof(float2), hipMemcpyHostToDevice));
+        ctx->tab_W = W;
+        ctx->tab_H = H;
+    }
+    if (ctx->wc_sigma != sigma_color || !ctx->wc) {
+        float h[256];
+        const double cf = 1.0 / (2.0 * (double)sigma_color * (double)sigma_color);
+        for (int a = 0; a < 256; ++a) h[a] = (float)exp(-(double)(a * a) * cf);
+        if (!ctx->wc) PMP_HIP(ctx, hipMalloc(&ctx->wc, 256 * sizeof(float)));
+        PMP_HIP(ctx, hipMemcpy(ctx->wc, h, sizeof(h), hipMemcpyHostToDevice));
+        ctx->wc_sigma = sigma_color;
+    }
+    return PMP_OK;
+}
+
+static void free_image(Image& im) {
+    if (im.gray) (void)hipFree(im.gray);
+    if (im.quad_wrap) (void)hipFree(im.quad_wrap);
+    if (im.quad_clamp) (void)hipFree(im.quad_clamp);
+    im = Image();
+}
+
+}  // namespace pmp
+
+using namespace pmp;
+
+extern "C" {
+
+int pmp_create(pmp_ctx** out, int32_t device) {
+    if (!out) return PMP_EINVAL;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return PMP_EDEVICE;
+    if (device < 0 || device >= n) return PMP_EINVAL;
+    pmp_ctx* ctx = new (std::nothrow) pmp_ctx();
+    if (!ctx) return PMP_ENOMEM;
+    ctx->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete ctx;
+        return PMP_EDEVICE;
+    }
+    *out = ctx;
+    return PMP_OK;
+}
+
+void pmp_destroy(pmp_ctx* ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto& kv : ctx->images) free_image(kv.second);
+    void* bufs[] = {ctx->work.plane, ctx->work.depth, ctx->work.score, ctx->init.plane, ctx->init.depth,
+                    ctx->init.score, ctx->out_plane, ctx->out_depth, ctx->out_score, ctx->out_nghbr,
+                    ctx->tmp0, ctx->tmp1, ctx->mask, ctx->rowtab, ctx->coltab, ctx->wc, ctx->counters,
+                    ctx->clean_maps, ctx->clean_poses, ctx->prune_z, ctx->prune_alive, ctx->prune_out,
+                    ctx->prune_pts, ctx->prune_cols, ctx->prune_kps, ctx->prune_blk, ctx->prune_poses, ctx->dbg};
+    for (void* b : bufs)
+        if (b) (void)hipFree(b);
+    for (auto& v : ctx->pruner) {
+        if (v.depth) (void)hipFree(v.depth);
+        if (v.bgr) (void)hipFree(v.bgr);
+    }
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char* pmp_last_error(const pmp_ctx* ctx) { return ctx ? ctx->last_error.c_str() : "null context"; }
+
+void pmp_default_params(pmp_params* p) {
+    if (!p) return;
+    memset(p, 0, sizeof(*p));
+    p->patch_half = 5;
+    p->patch_stride = 2;
+    p->iterations = 2;
+    p->n_refine = 6;
+    p->agg_mode = PMP_AGG_REF_FOLD;
+    p->topk = 2;
+    p->edge_mode = PMP_EDGE_WRAP;
+    p->coverage_mode = PMP_COVER_REF_GRID;
+    p->init_mode = PMP_INIT_REF_UNIFORM;
+    p->postprocess = 1;
+    p->seed = 0;
+    p->min_depth = 1e-5f;
+    p->max_depth = 10.0f;
+    p->init_inv_depth_min = 1.0f / 20.0f;
+    p->init_inv_depth_max = 1.0f / 0.3f;
+    p->min_score = 0.1f;
+    p->sigma_color = 50.0f;
+    p->sigma_space = 5.0f;
+    p->var_floor = 0.1f;
+    p->fold_agree = 0.25f;
+    p->depth_penalty = 1250.0f;
+    p->plane_penalty = 8.0f;
+    p->refine_depth_range = 0.02f;
+    p->refine_normal_range = 0.5f;
+    p->refine_depth_decay = 0.3f;
+    p->refine_normal_decay = 0.8f;
+    p->min_patch_sd = 0.0f;
+    p->median_rel_thr = 0.05f;
+}
+
+int pmp_synchronize(pmp_ctx* ctx) {
+    if (!ctx) return PMP_EINVAL;
+    PMP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return PMP_OK;
+}
+
+int pmp_image_put(pmp_ctx* ctx, uint64_t id, const uint8_t* pixels, int32_t width, int32_t height, int64_t pitch,
+                  int32_t on_device) {
+    if (!ctx || !pixels || width <= 0 || height <= 0 || pitch < width) return PMP_EINVAL;
+    PMP_HIP(ctx, hipSetDevice(ctx->device));
+    auto it = ctx->images.find(id);
+    if (it != ctx->images.end()) {
+        PMP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        free_image(it->second);
+        ctx->images.erase(it);
+    }
+    Image im;
+    im.W = width;
+    im.H = height;
+    const size_t bytes = (size_t)width * height;
+    PMP_HIP(ctx, hipMalloc(&im.gray, bytes));
+    im.bytes = bytes;
+    hipError_t e = hipMemcpy2DAsync(im.gray, width, pixels, (size_t)pitch, width, height,
+                                    on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+        free_image(im);
+        return check_hip(ctx, e, "pmp_image_put copy");
+    }
+    ctx->images[id] = im;
+    return PMP_OK;
+}
+
+int pmp_image_drop(pmp_ctx* ctx, uint64_t id) {
+    if (!ctx) return PMP_EINVAL;
+    auto it = ctx->images.find(id);
+    if (it == ctx->images.end()) return set_error(ctx, PMP_ENOTFOUND, "unknown image id");
+    PMP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    free_image(it->second);
+    ctx->images.erase(it);
+    if (ctx->views_set) {
+        bool used = ctx->ref_id == id;
+        for (uint64_t v : ctx->view_ids) used = used || v == id;
+        if (used) ctx->views_set = false;
+    }
+    return PMP_OK;
+}
+
+int pmp_image_pool_stats(const pmp_ctx* ctx, int64_t* count, int64_t* bytes) {
+    if (!ctx) return PMP_EINVAL;
+    int64_t b = 0;
+    for (auto& kv : ctx->images) b += (int64_t)kv.second.bytes;
+    if (count) *count = (int64_t)ctx->images.size();
+    if (bytes) *bytes = b;
+    return PMP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,531 @@
+// pmp_estimator.hip -- keyframe densification kernels and their C-ABI.
+//
+// Kernels (one launch each unless noted):
+//   k_pack        uint8 image -> 2x2-neighbourhood words (one 32-bit gather per
+//                 bilinear sample)                                  HBM-bound
+//   k_init        random hypothesis + initial score + snapshot   VALU-bound
+//                 (PatchMatchCuda.cu:210-270)
+//   k_score       score of the working plane map (PatchMatchCuda.cu:255-270)
+//   k_propagate   one red-black pass (PatchMatchCuda.cu:273-407), 2 per
+//                 iteration                                        VALU-bound
+//   k_copyback    copy-back window / gate (depthmap.cc:673-687, :731-760)
+//   k_median3     3x3 median, replicate border (cv::medianBlur), 2 launches
+//   k_postfilter  PostProcess test (depthmap.cc:1176-1198)            HBM-bound
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <string>
+
+#include "pmp_internal.h"
+#include "pmp_math.h"
+#include "pmp_score.h"
+
+namespace pmp {
+
+constexpr int kBlockX = 64;   // one wave across a row of same-colour pixels
+constexpr int kBlockY = 4;
+
+__device__ __forceinline__ void load_wc(const KfArgs& a, float* sWc) {
+    const int tid = threadIdx.y * blockDim.x + threadIdx.x;
+    for (int k = tid; k < 256; k += blockDim.x * blockDim.y) sWc[k] = a.Wc[k];
+    __syncthreads();
+}
+
+__device__ __forceinline__ void count_add(unsigned long long* ctr, unsigned int v) {
+    // one atomic per wave (all 64 lanes active; per-lane counts are < 2^16)
+    unsigned int s = v;
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    if ((threadIdx.x & 63) == 0 && s) atomicAdd(ctr, (unsigned long long)s);
+}
+
+// ---------------------------------------------------------------- pack
+__global__ void k_pack(const uint8_t* __restrict__ g, uint32_t* __restrict__ q, int W, int H, int wrap) {
+    int x = blockIdx.x * blockDim.x + threadIdx.x;
+    int y = blockIdx.y;
+    if (x >= W || y >= H) return;
+    int x1 = x + 1 < W ? x + 1 : (wrap ? 0 : W - 1);
+    int y1 = y + 1 < H ? y + 1 : H - 1;
+    const uint8_t* r0 = g + (size_t)y * W;
+    const uint8_t* r1 = g + (size_t)y1 * W;
+    q[(size_t)y * W + x] = (uint32_t)r0[x] | ((uint32_t)r0[x1] << 8) | ((uint32_t)r1[x] << 16) |
+                           ((uint32_t)r1[x1] << 24);
+}
+
+// ---------------------------------------------------------------- init
+template <int NV>
+__global__ __launch_bounds__(256) void k_init(KfArgs a, float4* __restrict__ plane, float* __restrict__ depth,
+                                              float* __restrict__ score, float4* __restrict__ iplane,
+                                              float* __restrict__ idepth, float* __restrict__ iscore) {
+    __shared__ float sWc[256];
+    load_wc(a, sWc);
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i >= a.H || j >= a.W) return;
+    float p0 = 0.0f, p1 = 0.0f, p2 = 0.0f, dep = 0.0f;
+    if (in_init_grid(a, i, j)) random_plane(a, i, j, p0, p1, p2, dep);
+    float sc = __uint_as_float(0xFFFFFFFFu);   // cudaMemset(score, 0xFF) (depthmap.cc:657)
+    if (in_score_grid(a, i, j)) {
+        RefStat rs = ref_stats(a, sWc, i, j);
+        sc = score_plane<NV>(a, sWc, rs, i, j, p0, p1, p2);
+    }
+    const size_t k = (size_t)i * a.W + j;
+    const float4 pl = make_float4(p0, p1, p2, 0.0f);
+    plane[k] = pl; depth[k] = dep; score[k] = sc;
+    iplane[k] = pl; idepth[k] = dep; iscore[k] = sc;
+}
+
+// ---------------------------------------------------------------- score field
+template <int NV>
+__global__ __launch_bounds__(256) void k_score(KfArgs a, const float4* __restrict__ plane,
+                                               float* __restrict__ score) {
+    __shared__ float sWc[256];
+    load_wc(a, sWc);
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i >= a.H || j >= a.W || !in_score_grid(a, i, j)) return;
+    const size_t k = (size_t)i * a.W + j;
+    RefStat rs = ref_stats(a, sWc, i, j);
+    const float4 p = plane[k];
+    score[k] = score_plane<NV>(a, sWc, rs, i, j, p.x, p.y, p.z);
+}
+
+// ---------------------------------------------------------------- red-black pass
+// Thread (jj, i) updates pixel j = 2 jj + ((i + colour) & 1): every pixel of
+// colour (i+j)&1 == colour once.  Candidates are at odd distance (1 or 3), so
+// they all have the other colour and are not written by this launch.
+template <int NV>
+__global__ __launch_bounds__(256) void k_propagate(KfArgs a, float4* __restrict__ plane,
+                                                   float* __restrict__ depth, float* __restrict__ score, int it,
+                                                   int colour, unsigned long long* __restrict__ counters) {
+    __shared__ float sWc[256];
+    load_wc(a, sWc);
+    const int i = blockIdx.y * blockDim.y + threadIdx.y;
+    const int j = 2 * (int)(blockIdx.x * blockDim.x + threadIdx.x) + ((i + colour) & 1);
+    unsigned int n_eval = 0, n_cand = 0;
+    if (i < a.H && j < a.W && in_prop_grid(a, i, j)) {
+        const int W = a.W;
+        const size_t k = (size_t)i * W + j;
+        const RefStat rs = ref_stats(a, sWc, i, j);
+        float sc = score[k];
+        const float4 pk = plane[k];
+        float h0 = pk.x, h1 = pk.y, h2 = pk.z;
+        float bx, by, bz;
+        bearing(a, i, j, bx, by, bz);
+        const float d0 = depth[k];
+        const float inv_d0 = rcp(d0);
+        // 1. propagation (PatchMatchCuda.cu:311-338)
+        // candidate order (0,+1) (+1,0) (0,-1) (-1,0) (0,+3) (+3,0) (0,-3) (-3,0)
+#pragma unroll 1
+        for (int c = 0; c < 8; ++c) {
+            const int dist = (c < 4) ? 1 : 3, dir = c & 3;
+            const int di = (dir == 1) ? dist : (dir == 3) ? -dist : 0;
+            const int dj = (dir == 0) ? dist : (dir == 2) ? -dist : 0;
+            const float4 pn = plane[(size_t)(i + di) * W + (j + dj)];
+            ++n_cand;
+            const float s = fma_(pn.z, bz, fma_(pn.y, by, pn.x * bx));
+            const float dn = rcp_pos(clampf(abs_(s), 1e-6f, 1e30f));
+            const float rel = (d0 - dn) * inv_d0;
+            const float fd = exp_((rel * rel) * -a.depth_penalty);
+            const float prod = fma_(h2, pn.z, fma_(h1, pn.y, h0 * pn.x));
+            const float ex = h0 - pn.x, ey = h1 - pn.y, ez = h2 - pn.z;
+            const float nrm = sqrt_(fma_(ez, ez, fma_(ey, ey, ex * ex)));
+            // acosf(prod/norm) is NaN, i.e. the candidate is rejected, unless |prod| <= norm
+            if (!(nrm > 0.0f && abs_(prod) <= nrm)) continue;
+            const float ratio = clampf(prod * rcp_pos(clampf(nrm, 1e-30f, 1e30f)), -1.0f, 1.0f);
+            const float ac = acos_(ratio);
+            const float fp = exp_((ac * ac) * -a.plane_penalty);
+            const float f = (1.0f - fd) * (1.0f - fp);
+            // |score| <= 1: the penalised score cannot exceed f
+            if (!(f > sc)) continue;
+            float sh = score_plane<NV>(a, sWc, rs, i, j, pn.x, pn.y, pn.z);
+            ++n_eval;
+            sh = sh * f;
+            if (sh > sc) { h0 = pn.x; h1 = pn.y; h2 = pn.z; sc = sh; }
+        }
+        // 2. refinement (PatchMatchCuda.cu:340-395)
+        float dcur = depth_of_plane(bx, by, bz, h0, h1, h2);
+        float c0 = h0, c1 = h1, c2 = h2;
+        float z[3 * kMaxRefine];
+        refine_normals<3 * kMaxRefine>(a, i, j, it, 3 * a.n_refine, z);
+#pragma unroll 1
+        for (int t = 0; t < a.n_refine; ++t) {
+            // z is indexed by the loop counter: select from the unrolled array
+            float ra = 0.0f, rb = 0.0f, rc = 0.0f;
+#pragma unroll
+            for (int u = 0; u < kMaxRefine; ++u)
+                if (u == t) { ra = z[3 * u]; rb = z[3 * u + 1]; rc = z[3 * u + 2]; }
+            const float dh = dcur * exp_(a.dr[t] * ra);
+            const float nx = fma_(a.nr[t], ra, c0);
+            const float ny = fma_(a.nr[t], rb, c1);
+            const float nz = fma_(a.nr[t], rc, c2);
+            float q0, q1, q2;
+            plane_from_depth_normal(bx, by, bz, dh, nx, ny, nz, q0, q1, q2);
+            const float sh = score_plane<NV>(a, sWc, rs, i, j, q0, q1, q2);
+            ++n_eval;
+            ++n_cand;
+            if (sh > sc) { c0 = q0; c1 = q1; c2 = q2; sc = sh; dcur = dh; }
+        }
+        score[k] = sc;
+        plane[k] = make_float4(c0, c1, c2, 0.0f);
+        depth[k] = depth_of_plane(bx, by, bz, c0, c1, c2);
+    }
+    count_add(&counters[0], n_eval);
+    count_add(&counters[1], n_cand);
+}
+
+// ---------------------------------------------------------------- finalize
+// Copy-back of ComputePatchMatch: masked pixels keep the AssignMatrices values
+// (depth 0, plane 0, score -1, nghbr 0); unmasked pixels take the maps after
+// RandomInitialization (depthmap.cc:673-687) unless they are inside the window
+// [ps-2, dim-ps-2) (depthmap.cc:731-732), where they take the final maps, or
+// 0 when the patch variance is below min_patch_sd^2 (depthmap.cc:739-743).
+__global__ void k_copyback(KfArgs a, const uint8_t* __restrict__ mask, const float4* __restrict__ ip,
+                           const float* __restrict__ id, const float* __restrict__ is,
+                           const float4* __restrict__ fp, const float* __restrict__ fd,
+                           const float* __restrict__ fs, float4* __restrict__ op, float* __restrict__ od,
+                           float* __restrict__ os, int32_t* __restrict__ on, float min_var, float inv_T) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.y;
+    if (j >= a.W) return;
+    const int W = a.W, H = a.H;
+    const size_t k = (size_t)i * W + j;
+    const int ps = a.half + 2;
+    const int lo = ps - 2, hi_r = H - ps - 2, hi_c = W - ps - 2;
+    float4 pl = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    float dep = 0.0f, sc = -1.0f;
+    int ng = 0;
+    if (mask == nullptr || mask[k]) {
+        if (i >= lo && i < hi_r && j >= lo && j < hi_c) {
+            bool low = false;
+            if (min_var > 0.0f) {
+                float sum = 0.0f;
+                for (int k1 = -a.half; k1 <= a.half; k1 += a.stride)
+                    for (int k2 = -a.half; k2 <= a.half; k2 += a.stride)
+                        sum = sum + (float)a.ref[(size_t)(i + k1) * W + (j + k2)];
+                const float mean = sum * inv_T;
+                float s2 = 0.0f;
+                for (int k1 = -a.half; k1 <= a.half; k1 += a.stride)
+                    for (int k2 = -a.half; k2 <= a.half; k2 += a.stride) {
+                        float e = (float)a.ref[(size_t)(i + k1) * W + (j + k2)] - mean;
+                        s2 = fma_(e, e, s2);
+                    }
+                low = (s2 * inv_T) < min_var;
+            }
+            if (!low) {
+                dep = fd[k]; sc = fs[k];
+                const float4 f = fp[k];
+                pl = make_float4(f.x, f.y, f.z, 0.0f);
+            } else {
+                sc = 0.0f;
+            }
+            ng = 1;
+        } else {
+            dep = id[k]; sc = is[k];
+            const float4 f = ip[k];
+            pl = make_float4(f.x, f.y, f.z, 0.0f);
+        }
+    }
+    op[k] = pl; od[k] = dep; os[k] = sc; on[k] = ng;
+}
+
+// 3x3 median with replicated border (cv::medianBlur, ksize 3, CV_32F)
+__global__ void k_median3(const float* __restrict__ src, float* __restrict__ dst, int W, int H) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.y;
+    if (j >= W) return;
+    float v[9];
+    int n = 0;
+#pragma unroll
+    for (int a = -1; a <= 1; ++a) {
+        const int ii = min(max(i + a, 0), H - 1);
+#pragma unroll
+        for (int b = -1; b <= 1; ++b) {
+            const int jj = min(max(j + b, 0), W - 1);
+            v[n++] = src[(size_t)ii * W + jj];
+        }
+    }
+    // median of 9 by a fixed exchange network (the value equals any sort's v[4])
+#pragma unroll
+    for (int p = 0; p <= 4; ++p)
+#pragma unroll
+        for (int b = p + 1; b < 9; ++b) {
+            const float lo = fminf(v[p], v[b]);
+            const float hi = fmaxf(v[p], v[b]);
+            v[p] = lo;
+            v[b] = hi;
+        }
+    dst[(size_t)i * W + j] = v[4];
+}
+
+// PostProcess test (depthmap.cc:1189-1196)
+__global__ void k_postfilter(const float* __restrict__ med, float* __restrict__ depth, float* __restrict__ score,
+                             int n, float rel_thr, float min_depth, float max_depth, float min_score) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const float d = depth[k], m = med[k], sc = score[k];
+    if (d == 0.0f || (double)(__builtin_fabsf(d - m) / d) > (double)rel_thr || d < min_depth || d > max_depth ||
+        sc <= min_score) {
+        depth[k] = 0.0f;
+        score[k] = -1.0f;
+    } else {
+        depth[k] = m;
+    }
+}
+
+// ---------------------------------------------------------------- host helpers
+static int fill_args(pmp_ctx* ctx, const pmp_params* p, KfArgs& a) {
+    if (!ctx->views_set) return set_error(ctx, PMP_ESTATE, "pmp_set_views has not been called");
+    if (p->patch_half < 3 || p->patch_stride < 1 || p->patch_half > 5 + 5)
+        return set_error(ctx, PMP_EINVAL, "patch_half must be >= 3 (propagation reads 3 pixels away)");
+    if (p->n_refine < 0 || p->n_refine > kMaxRefine) return set_error(ctx, PMP_EINVAL, "n_refine must be in [0, 8]");
+    if (p->agg_mode < 0 || p->agg_mode > 2 || p->edge_mode < 0 || p->edge_mode > 2 || p->coverage_mode < 0 ||
+        p->coverage_mode > 1 || p->init_mode < 0 || p->init_mode > 1)
+        return set_error(ctx, PMP_EINVAL, "unknown mode value");
+    if (!(p->sigma_color > 0.0f) || !(p->sigma_space > 0.0f))
+        return set_error(ctx, PMP_EINVAL, "sigma_color and sigma_space must be positive");
+    memset(&a, 0, sizeof(a));
+    a.W = ctx->W; a.H = ctx->H; a.N = (int)ctx->view_ids.size();
+    a.half = p->patch_half; a.stride = p->patch_stride;
+    int T = 0;
+    for (int k1 = -a.half; k1 <= a.half; k1 += a.stride)
+        for (int k2 = -a.half; k2 <= a.half; k2 += a.stride) {
+            if (T >= kMaxTaps) return set_error(ctx, PMP_EINVAL, "too many taps (max 121)");
+            a.Ws[T++] = (float)exp(-(double)(k1 * k1 + k2 * k2) /
+                                   (2.0 * (double)p->sigma_space * (double)p->sigma_space));
+        }
+    a.T = T;
+    if (a.W < 2 * a.half + 8 || a.H < 2 * a.half + 8) return set_error(ctx, PMP_EINVAL, "image too small for the patch");
+    a.agg_mode = p->agg_mode; a.edge_mode = p->edge_mode; a.coverage = p->coverage_mode;
+    a.init_mode = p->init_mode; a.n_refine = p->n_refine;
+    int k = p->topk < 1 ? 1 : p->topk;
+    if (k > a.N) k = a.N;
+    a.topk = k;
+    a.inv_k = (float)(1.0 / (double)k);
+    int rc = ensure_tables(ctx, a.W, a.H, p->sigma_color);
+    if (rc) return rc;
+    a.ref = ctx->images[ctx->ref_id].gray;
+    for (int v = 0; v < a.N; ++v) {
+        Image& im = ctx->images[ctx->view_ids[v]];
+        uint32_t*& q = (p->edge_mode == PMP_EDGE_WRAP) ? im.quad_wrap : im.quad_clamp;
+        if (!q) {
+            size_t bytes = (size_t)im.W * im.H * 4;
+            PMP_HIP(ctx, hipMalloc(&q, bytes));
+            im.bytes += bytes;
+            dim3 blk(256), grd((im.W + 255) / 256, im.H);
+            hipLaunchKernelGGL(k_pack, grd, blk, 0, ctx->stream, im.gray, q, im.W, im.H,
+                               p->edge_mode == PMP_EDGE_WRAP ? 1 : 0);
+            PMP_HIP(ctx, hipGetLastError());
+        }
+        a.quads[v] = q;
+    }
+    a.rowtab = ctx->rowtab; a.coltab = ctx->coltab; a.Wc = ctx->wc;
+    derive_poses(ctx->ref_pose, a.N, ctx->view_poses.data(), a.M, a.c);
+    a.ku = (float)((double)a.W / (2.0 * M_PI));
+    a.hu = (float)(0.5 * a.W);
+    a.kv = (float)((double)a.H / M_PI);
+    a.hv = (float)(0.5 * a.H);
+    a.var_floor = p->var_floor; a.fold_agree = p->fold_agree;
+    a.depth_penalty = p->depth_penalty; a.plane_penalty = p->plane_penalty;
+    float dr = p->refine_depth_range, nr = p->refine_normal_range;
+    for (int t = 0; t < kMaxRefine; ++t) {
+        a.dr[t] = dr; a.nr[t] = nr;
+        dr = (float)((double)dr * (double)p->refine_depth_decay);
+        nr = (float)((double)nr * (double)p->refine_normal_decay);
+    }
+    a.min_depth = p->min_depth; a.max_depth = p->max_depth;
+    a.inv_lo = p->init_inv_depth_min; a.inv_hi = p->init_inv_depth_max;
+    a.key0 = (uint32_t)(p->seed & 0xFFFFFFFFu);
+    a.key1 = (uint32_t)(p->seed >> 32);
+    return PMP_OK;
+}
+
+#define PMP_DISPATCH_NV(N, KERNEL, GRID, BLOCK, ...)                                          \
+    switch (N) {                                                                             \
+        case 1: hipLaunchKernelGGL(KERNEL<1>, GRID, BLOCK, 0, ctx->stream, __VA_ARGS__); break;   \
+        case 2: hipLaunchKernelGGL(KERNEL<2>, GRID, BLOCK, 0, ctx->stream, __VA_ARGS__); break;   \
+        case 3: hipLaunchKernelGGL(KERNEL<3>, GRID, BLOCK, 0, ctx->stream, __VA_ARGS__); break;   \
+        case 4: hipLaunchKernelGGL(KERNEL<4>, GRID, BLOCK, 0, ctx->stream, __VA_ARGS__); break;   \
+        case 5: hipLaunchKernelGGL(KERNEL<5>, GRID, BLOCK, 0, ctx->stream, __VA_ARGS__); break;   \
+        case 6: hipLaunchKernelGGL(KERNEL<6>, GRID, BLOCK, 0, ctx->stream, __VA_ARGS__); break;   \
+        case 7: hipLaunchKernelGGL(KERNEL<7>, GRID, BLOCK, 0, ctx->stream, __VA_ARGS__); break;   \
+        case 8: hipLaunchKernelGGL(KERNEL<8>, GRID, BLOCK, 0, ctx->stream, __VA_ARGS__); break;   \
+        case 9: hipLaunchKernelGGL(KERNEL<9>, GRID, BLOCK, 0, ctx->stream, __VA_ARGS__); break;   \
+        case 10: hipLaunchKernelGGL(KERNEL<10>, GRID, BLOCK, 0, ctx->stream, __VA_ARGS__); break; \
+        default: return set_error(ctx, PMP_EINVAL, "n_views must be in [1, 10]");            \
+    }
+
+static int launch_init(pmp_ctx* ctx, const KfArgs& a) {
+    dim3 blk(kBlockX, kBlockY), grd((a.W + kBlockX - 1) / kBlockX, (a.H + kBlockY - 1) / kBlockY);
+    PMP_DISPATCH_NV(a.N, k_init, grd, blk, a, ctx->work.plane, ctx->work.depth, ctx->work.score, ctx->init.plane,
+                    ctx->init.depth, ctx->init.score);
+    PMP_HIP(ctx, hipGetLastError());
+    return PMP_OK;
+}
+
+static int launch_score(pmp_ctx* ctx, const KfArgs& a) {
+    dim3 blk(kBlockX, kBlockY), grd((a.W + kBlockX - 1) / kBlockX, (a.H + kBlockY - 1) / kBlockY);
+    PMP_DISPATCH_NV(a.N, k_score, grd, blk, a, ctx->work.plane, ctx->work.score);
+    PMP_HIP(ctx, hipGetLastError());
+    return PMP_OK;
+}
+
+static int launch_passes(pmp_ctx* ctx, const KfArgs& a, int it_begin, int it_end) {
+    PMP_HIP(ctx, hipMemsetAsync(ctx->counters, 0, 2 * sizeof(unsigned long long), ctx->stream));
+    const int half_w = (a.W + 1) / 2;
+    dim3 blk(kBlockX, kBlockY), grd((half_w + kBlockX - 1) / kBlockX, (a.H + kBlockY - 1) / kBlockY);
+    for (int it = it_begin; it < it_end; ++it)
+        for (int colour = 0; colour < 2; ++colour) {
+            PMP_DISPATCH_NV(a.N, k_propagate, grd, blk, a, ctx->work.plane, ctx->work.depth, ctx->work.score, it,
+                            colour, ctx->counters);
+            PMP_HIP(ctx, hipGetLastError());
+        }
+    return PMP_OK;
+}
+
+static int launch_finalize(pmp_ctx* ctx, const KfArgs& a, const pmp_params* p, const uint8_t* mask_host) {
+    const size_t P = (size_t)a.W * a.H;
+    const uint8_t* dmask = nullptr;
+    if (mask_host) {
+        PMP_HIP(ctx, hipMemcpyAsync(ctx->mask, mask_host, P, hipMemcpyHostToDevice, ctx->stream));
+        dmask = ctx->mask;
+    }
+    dim3 blk(256), grd((a.W + 255) / 256, a.H);
+    hipLaunchKernelGGL(k_copyback, grd, blk, 0, ctx->stream, a, dmask, ctx->init.plane, ctx->init.depth,
+                       ctx->init.score, ctx->work.plane, ctx->work.depth, ctx->work.score, ctx->out_plane,
+                       ctx->out_depth, ctx->out_score, ctx->out_nghbr, p->min_patch_sd * p->min_patch_sd,
+                       (float)(1.0 / (double)a.T));
+    PMP_HIP(ctx, hipGetLastError());
+    if (p->postprocess) {
+        hipLaunchKernelGGL(k_median3, grd, blk, 0, ctx->stream, ctx->out_depth, ctx->tmp0, a.W, a.H);
+        hipLaunchKernelGGL(k_median3, grd, blk, 0, ctx->stream, ctx->tmp0, ctx->tmp1, a.W, a.H);
+        hipLaunchKernelGGL(k_postfilter, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, ctx->stream, ctx->tmp1,
+                           ctx->out_depth, ctx->out_score, (int)P, p->median_rel_thr, p->min_depth, p->max_depth,
+                           p->min_score);
+        PMP_HIP(ctx, hipGetLastError());
+    }
+    return PMP_OK;
+}
+
+}  // namespace pmp
+
+using namespace pmp;
+
+extern "C" {
+
+int pmp_set_views(pmp_ctx* ctx, uint64_t ref_id, const pmp_pose* ref_pose, int32_t n_views, const uint64_t* view_ids,
+                  const pmp_pose* view_poses) {
+    if (!ctx || !ref_pose || (n_views > 0 && (!view_ids || !view_poses))) return PMP_EINVAL;
+    if (n_views < 1 || n_views > kMaxViews) return set_error(ctx, PMP_EINVAL, "n_views must be in [1, 10]");
+    auto it = ctx->images.find(ref_id);
+    if (it == ctx->images.end()) return set_error(ctx, PMP_ENOTFOUND, "unknown reference image id");
+    const int W = it->second.W, H = it->second.H;
+    for (int v = 0; v < n_views; ++v) {
+        auto iv = ctx->images.find(view_ids[v]);
+        if (iv == ctx->images.end()) return set_error(ctx, PMP_ENOTFOUND, "unknown neighbour image id");
+        if (iv->second.W != W || iv->second.H != H)
+            return set_error(ctx, PMP_EINVAL, "neighbour images must have the reference's size");
+    }
+    int rc = ensure_workspace(ctx, W, H);
+    if (rc) return rc;
+    ctx->ref_id = ref_id;
+    ctx->view_ids.assign(view_ids, view_ids + n_views);
+    memcpy(ctx->ref_pose, ref_pose->R, 9 * sizeof(double));
+    memcpy(ctx->ref_pose + 9, ref_pose->t, 3 * sizeof(double));
+    ctx->view_poses.resize(12 * (size_t)n_views);
+    for (int v = 0; v < n_views; ++v) {
+        memcpy(&ctx->view_poses[12 * v], view_poses[v].R, 9 * sizeof(double));
+        memcpy(&ctx->view_poses[12 * v + 9], view_poses[v].t, 3 * sizeof(double));
+    }
+    ctx->W = W; ctx->H = H;
+    ctx->views_set = true;
+    return PMP_OK;
+}
+
+int pmp_random_initialization(pmp_ctx* ctx, const pmp_params* params) {
+    if (!ctx || !params) return PMP_EINVAL;
+    KfArgs a;
+    int rc = fill_args(ctx, params, a);
+    if (rc) return rc;
+    return launch_init(ctx, a);
+}
+
+int pmp_score_plane_depth(pmp_ctx* ctx, const pmp_params* params) {
+    if (!ctx || !params) return PMP_EINVAL;
+    KfArgs a;
+    int rc = fill_args(ctx, params, a);
+    if (rc) return rc;
+    return launch_score(ctx, a);
+}
+
+int pmp_red_black_pass(pmp_ctx* ctx, const pmp_params* params, int32_t it_begin, int32_t it_end) {
+    if (!ctx || !params || it_begin < 0 || it_end < it_begin) return PMP_EINVAL;
+    KfArgs a;
+    int rc = fill_args(ctx, params, a);
+    if (rc) return rc;
+    return launch_passes(ctx, a, it_begin, it_end);
+}
+
+int pmp_finalize(pmp_ctx* ctx, const pmp_params* params, const uint8_t* mask) {
+    if (!ctx || !params) return PMP_EINVAL;
+    KfArgs a;
+    int rc = fill_args(ctx, params, a);
+    if (rc) return rc;
+    return launch_finalize(ctx, a, params, mask);
+}
+
+int pmp_densify(pmp_ctx* ctx, const pmp_params* params, const uint8_t* mask) {
+    if (!ctx || !params || params->iterations < 0) return PMP_EINVAL;
+    KfArgs a;
+    int rc = fill_args(ctx, params, a);
+    if (rc) return rc;
+    if ((rc = launch_init(ctx, a))) return rc;
+    if ((rc = launch_passes(ctx, a, 0, params->iterations))) return rc;
+    return launch_finalize(ctx, a, params, mask);
+}
+
+int pmp_set_maps(pmp_ctx* ctx, const float* plane4, const float* depth, const float* score) {
+    if (!ctx) return PMP_EINVAL;
+    if (!ctx->views_set) return set_error(ctx, PMP_ESTATE, "pmp_set_views has not been called");
+    const size_t P = (size_t)ctx->W * ctx->H;
+    if (plane4) PMP_HIP(ctx, hipMemcpyAsync(ctx->work.plane, plane4, P * 16, hipMemcpyHostToDevice, ctx->stream));
+    if (depth) PMP_HIP(ctx, hipMemcpyAsync(ctx->work.depth, depth, P * 4, hipMemcpyHostToDevice, ctx->stream));
+    if (score) PMP_HIP(ctx, hipMemcpyAsync(ctx->work.score, score, P * 4, hipMemcpyHostToDevice, ctx->stream));
+    PMP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return PMP_OK;
+}
+
+int pmp_get_maps(pmp_ctx* ctx, int32_t which, float* plane4, float* depth, float* score, int32_t* nghbr) {
+    if (!ctx) return PMP_EINVAL;
+    if (!ctx->views_set) return set_error(ctx, PMP_ESTATE, "pmp_set_views has not been called");
+    const size_t P = (size_t)ctx->W * ctx->H;
+    const void *sp, *sd, *ss;
+    if (which == PMP_MAPS_WORK) { sp = ctx->work.plane; sd = ctx->work.depth; ss = ctx->work.score; }
+    else if (which == PMP_MAPS_INIT) { sp = ctx->init.plane; sd = ctx->init.depth; ss = ctx->init.score; }
+    else if (which == PMP_MAPS_FINAL) { sp = ctx->out_plane; sd = ctx->out_depth; ss = ctx->out_score; }
+    else return set_error(ctx, PMP_EINVAL, "unknown map set");
+    if (plane4) PMP_HIP(ctx, hipMemcpyAsync(plane4, sp, P * 16, hipMemcpyDeviceToHost, ctx->stream));
+    if (depth) PMP_HIP(ctx, hipMemcpyAsync(depth, sd, P * 4, hipMemcpyDeviceToHost, ctx->stream));
+    if (score) PMP_HIP(ctx, hipMemcpyAsync(score, ss, P * 4, hipMemcpyDeviceToHost, ctx->stream));
+    if (nghbr) {
+        if (which != PMP_MAPS_FINAL) return set_error(ctx, PMP_EINVAL, "nghbr exists only for the final maps");
+        PMP_HIP(ctx, hipMemcpyAsync(nghbr, ctx->out_nghbr, P * 4, hipMemcpyDeviceToHost, ctx->stream));
+    }
+    PMP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return PMP_OK;
+}
+
+int pmp_eval_counts(pmp_ctx* ctx, int64_t* evals, int64_t* candidates) {
+    if (!ctx) return PMP_EINVAL;
+    unsigned long long h[2] = {0, 0};
+    if (ctx->counters) {
+        PMP_HIP(ctx, hipMemcpyAsync(h, ctx->counters, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+        PMP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    if (evals) *evals = (int64_t)h[0];
+    if (candidates) *candidates = (int64_t)h[1];
+    return PMP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,288 @@
+// pmp_score.h -- device side of the plane-hypothesis score and its helpers.
+//
+// The score follows ComputeScorePlaneHypothesis (PatchMatchCuda.cu:46-180):
+// for every tap (k1 outer, k2 inner) the reference bearing is intersected
+// with the plane, moved into each neighbour camera, projected to
+// equirectangular (u, v), sampled bilinearly, and accumulated into weighted
+// moments; each view gives an NCC; views are aggregated.  The arithmetic is
+// the canonical FP32 sequence of this project (identical, operation for
+// operation, to the host restatement):
+//   * bearings from separable row/column tables;
+//   * one fused rigid transform per view (M, c);
+//   * weights Wc[|dI|] * Ws[tap] from tables;
+//   * shifted moments (reference taps centred on their weighted mean,
+//     neighbour samples shifted by the view's first-tap sample);
+//   * quad-packed uint8 neighbours: one 32-bit load per tap and view.
+#pragma once
+#include "pmp_internal.h"
+#include "pmp_math.h"
+
+namespace pmp {
+
+// ---- coverage (PatchMatchCuda.cu:262,283; grids :465-470, :538-539, :578-581)
+PMP_DF bool interior(const KfArgs& a, int i, int j) {
+    int ph = a.half;
+    return !(i < ph + 1 || i > a.H - ph - 1 || j < ph + 1 || j > a.W - ph - 1);
+}
+PMP_DF bool in_init_grid(const KfArgs& a, int i, int j) {
+    if (a.coverage == PMP_COVER_FULL) return true;
+    return i < (a.H / 16) * 16 && j < (a.W / 16) * 16;
+}
+PMP_DF bool in_score_grid(const KfArgs& a, int i, int j) {
+    if (!interior(a, i, j)) return false;
+    if (a.coverage == PMP_COVER_FULL) return true;
+    return i < (a.H / 32) * 32 && j < (a.W / 32) * 32;
+}
+PMP_DF bool in_prop_grid(const KfArgs& a, int i, int j) {
+    if (!interior(a, i, j)) return false;
+    if (a.coverage == PMP_COVER_FULL) return true;
+    int J = 32 * ((a.W / 32) / 2);
+    return i < (a.H / 32) * 32 && (j + (j & 1)) < 2 * J;
+}
+
+PMP_DF void bearing(const KfArgs& a, int i, int j, float& bx, float& by, float& bz) {
+    float2 r = a.rowtab[i];
+    float2 c = a.coltab[j];
+    bx = r.x * c.x;
+    by = -r.y;
+    bz = r.x * c.y;
+}
+
+struct RefStat {
+    float Ic, mx, inv, mxc, vx;
+};
+
+// Plane-independent reference-side moments of the patch (two passes).
+PMP_DF RefStat ref_stats(const KfArgs& a, const float* sWc, int i, int j) {
+    RefStat s;
+    const int W = a.W;
+    s.Ic = (float)a.ref[(size_t)i * W + j];
+    float Sw = 0.0f, Swx = 0.0f;
+    int t = 0;
+    for (int k1 = -a.half; k1 <= a.half; k1 += a.stride) {
+        const uint8_t* row = a.ref + (size_t)(i + k1) * W + j;
+        for (int k2 = -a.half; k2 <= a.half; k2 += a.stride, ++t) {
+            float xs = (float)row[k2] - s.Ic;
+            float w = sWc[(int)abs_(xs)] * a.Ws[t];
+            Sw = Sw + w;
+            Swx = fma_(w, xs, Swx);
+        }
+    }
+    s.inv = rcp_pos(Sw);
+    s.mx = Swx * s.inv;
+    float Sc = 0.0f, Scc = 0.0f;
+    t = 0;
+    for (int k1 = -a.half; k1 <= a.half; k1 += a.stride) {
+        const uint8_t* row = a.ref + (size_t)(i + k1) * W + j;
+        for (int k2 = -a.half; k2 <= a.half; k2 += a.stride, ++t) {
+            float xs = (float)row[k2] - s.Ic;
+            float w = sWc[(int)abs_(xs)] * a.Ws[t];
+            float xc = xs - s.mx;
+            float wxc = w * xc;
+            Sc = Sc + wxc;
+            Scc = fma_(wxc, xc, Scc);
+        }
+    }
+    s.mxc = Sc * s.inv;
+    s.vx = fma_(-s.mxc, s.mxc, Scc * s.inv);
+    return s;
+}
+
+PMP_DF float ncc(const RefStat& rs, float floor_, float Swy, float Swyy, float Swxy) {
+    float my = Swy * rs.inv;
+    float vy = fma_(-my, my, Swyy * rs.inv);
+    float cov = fma_(-rs.mxc, my, Swxy * rs.inv);
+    if (rs.vx < floor_ || vy < floor_) return -1.0f;
+    float s = cov * rsqrt_pos(rs.vx * vy);
+    return clampf(s, -1.0f, 1.0f);
+}
+
+// Neighbour sample at (u, v); texel k at integer k (PatchMatchCuda.cu:146 looks
+// up (u+0.5, v+0.5) with texel centres at k+0.5).
+PMP_DF float sample(const uint32_t* __restrict__ q, int W, int H, int edge, float u, float v) {
+    float fx, fy;
+    uint32_t word;
+    if (edge == PMP_EDGE_ZERO) {
+        if (!(u >= 0.0f && u < (float)(W - 1) && v >= 0.0f && v < (float)(H - 1))) return 0.0f;
+        float x0f = floor_(u), y0f = floor_(v);
+        fx = u - x0f;
+        fy = v - y0f;
+        word = q[(size_t)(int)y0f * W + (int)x0f];
+    } else {
+        float uu = clampf(u, 0.0f, edge == PMP_EDGE_WRAP ? (float)W : (float)(W - 1));
+        float vv = clampf(v, 0.0f, (float)(H - 1));
+        float x0f = floor_(uu), y0f = floor_(vv);
+        fx = uu - x0f;
+        fy = vv - y0f;
+        int x0 = (int)x0f;
+        if (x0 >= W) x0 -= W;
+        word = q[(size_t)(int)y0f * W + x0];
+    }
+    float b0 = (float)(word & 0xFFu);
+    float b1 = (float)((word >> 8) & 0xFFu);
+    float b2 = (float)((word >> 16) & 0xFFu);
+    float b3 = (float)(word >> 24);
+    float top = fma_(fx, b1 - b0, b0);
+    float bot = fma_(fx, b3 - b2, b2);
+    return fma_(fy, bot - top, top);
+}
+
+// Equirectangular projection (PatchMatchCuda.cu:132-144).
+PMP_DF void project(const KfArgs& a, float x, float y, float z, float& u, float& v) {
+    float r2 = fma_(x, x, fma_(y, y, z * z));
+    r2 = clampf(r2, 1e-12f, 1e30f);
+    float ir = rsqrt_pos(r2);
+    float sy = clampf(y * ir, -1.0f, 1.0f);
+    float lat = asin_(sy);
+    float lon = atan2_(x, z);
+    u = fma_(lon, a.ku, a.hu);
+    v = fma_(lat, a.kv, a.hv);
+}
+
+template <int NV>
+PMP_DF float aggregate(const KfArgs& a, float (&sv)[NV]) {
+    if (a.agg_mode == PMP_AGG_MAX) {
+        float s = -1.0f;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) s = (sv[v] > s) ? sv[v] : s;
+        return s;
+    }
+    if (a.agg_mode == PMP_AGG_TOPK_MEAN) {
+        // exchange sort: position p ends with the max of positions p..NV-1,
+        // so positions 0..k-1 hold the k largest in descending order
+        float sum = 0.0f;
+#pragma unroll
+        for (int p = 0; p < NV; ++p) {
+#pragma unroll
+            for (int b = p + 1; b < NV; ++b) {
+                float hi = (sv[b] > sv[p]) ? sv[b] : sv[p];
+                float lo = (sv[b] > sv[p]) ? sv[p] : sv[b];
+                sv[p] = hi;
+                sv[b] = lo;
+            }
+            if (p < a.topk) sum = (p == 0) ? sv[0] : sum + sv[p];
+        }
+        return sum * a.inv_k;
+    }
+    float s = -1.0f;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        float s2 = sv[v];
+        if (abs_(s - s2) < a.fold_agree * abs_(s2)) s = 0.5f * (s + s2);
+        else s = (s2 > s) ? s2 : s;
+    }
+    return s;
+}
+
+// Score of plane (px, py, pz) at pixel (i, j).
+template <int NV>
+PMP_DF float score_plane(const KfArgs& a, const float* sWc, const RefStat& rs, int i, int j, float px, float py,
+                         float pz) {
+    const int W = a.W, H = a.H;
+    float acc0[NV], acc1[NV], acc2[NV], shift[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) { acc0[v] = 0.0f; acc1[v] = 0.0f; acc2[v] = 0.0f; shift[v] = 0.0f; }
+    int t = 0;
+    for (int k1 = -a.half; k1 <= a.half; k1 += a.stride) {
+        const int ii = i + k1;
+        const float2 rt = a.rowtab[ii];
+        const uint8_t* row = a.ref + (size_t)ii * W + j;
+        for (int k2 = -a.half; k2 <= a.half; k2 += a.stride, ++t) {
+            const float2 ct = a.coltab[j + k2];
+            const float bx = rt.x * ct.x, by = -rt.y, bz = rt.x * ct.y;
+            const float s = fma_(pz, bz, fma_(py, by, px * bx));
+            float rsv = rcp_pos(clampf(abs_(s), 1e-6f, 1e30f));
+            if (s < 0.0f) rsv = -rsv;
+            const float xc = bx * rsv, yc = by * rsv, zc = bz * rsv;
+            const float xs = (float)row[k2] - rs.Ic;
+            const float w = sWc[(int)abs_(xs)] * a.Ws[t];
+            const float wxc = w * (xs - rs.mx);
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                const float x = fma_(a.M[v][0], xc, fma_(a.M[v][1], yc, fma_(a.M[v][2], zc, a.c[v][0])));
+                const float y = fma_(a.M[v][3], xc, fma_(a.M[v][4], yc, fma_(a.M[v][5], zc, a.c[v][1])));
+                const float z = fma_(a.M[v][6], xc, fma_(a.M[v][7], yc, fma_(a.M[v][8], zc, a.c[v][2])));
+                float u, vv;
+                project(a, x, y, z, u, vv);
+                const float val = sample(a.quads[v], W, H, a.edge_mode, u, vv);
+                if (t == 0) shift[v] = val;
+                const float yd = val - shift[v];
+                const float tw = w * yd;
+                acc0[v] = acc0[v] + tw;
+                acc1[v] = fma_(tw, yd, acc1[v]);
+                acc2[v] = fma_(wxc, yd, acc2[v]);
+            }
+        }
+    }
+    float sv[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) sv[v] = ncc(rs, a.var_floor, acc0[v], acc1[v], acc2[v]);
+    return aggregate<NV>(a, sv);
+}
+
+// PlaneFromDepthNormal (PatchMatchCuda.cu:31-43)
+PMP_DF void plane_from_depth_normal(float bx, float by, float bz, float dep, float nx, float ny, float nz,
+                                    float& p0, float& p1, float& p2) {
+    float qx = dep * bx, qy = dep * by, qz = dep * bz;
+    float den = fma_(nz, qz, fma_(ny, qy, nx * qx));
+    float r = rcp(den);
+    p0 = nx * r;
+    p1 = ny * r;
+    p2 = nz * r;
+}
+
+// 1/fmaxf(1e-6, |b.p|) (PatchMatchCuda.cu:340-341, :402-403)
+PMP_DF float depth_of_plane(float bx, float by, float bz, float p0, float p1, float p2) {
+    float s = fma_(p2, bz, fma_(p1, by, p0 * bx));
+    return rcp_pos(clampf(abs_(s), 1e-6f, 1e30f));
+}
+
+// Random hypothesis of one pixel (ComputeRandomDepthmap_/ComputeRandomPlanemap_,
+// PatchMatchCuda.cu:210-244)
+PMP_DF void random_plane(const KfArgs& a, int i, int j, float& p0, float& p1, float& p2, float& dep) {
+    u32x4 r = philox((uint32_t)(i * a.W + j), 0xFFFFFFFFu, 0u, kStreamInit, a.key0, a.key1);
+    float bx, by, bz;
+    bearing(a, i, j, bx, by, bz);
+    float nx, ny, nz;
+    if (a.init_mode == PMP_INIT_INVDEPTH_HEMI) {
+        float ud = u01_oc(r.x);
+        float inv = fma_(ud, a.inv_hi - a.inv_lo, a.inv_lo);
+        dep = rcp_pos(clampf(inv, 1e-30f, 1e30f));
+        float zc = fma_(2.0f, u01_co(r.y), -1.0f);
+        float rr = sqrt_(fma_(-zc, zc, 1.0f));
+        float s, c;
+        sincos2pi(u01_co(r.z), &s, &c);
+        nx = rr * c;
+        ny = rr * s;
+        nz = zc;
+        float dn = fma_(nz, bz, fma_(ny, by, nx * bx));
+        if (dn > 0.0f) { nx = -nx; ny = -ny; nz = -nz; }
+    } else {
+        dep = fma_(u01_oc(r.x), a.max_depth - a.min_depth, a.min_depth);
+        nx = fma_(u01_oc(r.y), 2.0f, -1.0f);
+        ny = fma_(u01_oc(r.z), 2.0f, -1.0f);
+        nz = fma_(u01_oc(r.w), 2.0f, -1.0f);
+    }
+    plane_from_depth_normal(bx, by, bz, dep, nx, ny, nz, p0, p1, p2);
+}
+
+// Standard normals n of (pixel, iteration): counter (pixel, it, block, "RFNE").
+template <int NMAX>
+PMP_DF void refine_normals(const KfArgs& a, int i, int j, int it, int n, float (&z)[NMAX]) {
+    const uint32_t pix = (uint32_t)(i * a.W + j);
+#pragma unroll
+    for (int b = 0; b < (NMAX + 3) / 4; ++b) {
+        if (4 * b < n) {
+            u32x4 r = philox(pix, (uint32_t)it, (uint32_t)b, kStreamRefine, a.key0, a.key1);
+            float z0, z1, z2, z3;
+            box_muller(r.x, r.y, &z0, &z1);
+            box_muller(r.z, r.w, &z2, &z3);
+            if (4 * b + 0 < NMAX) z[4 * b + 0] = z0;
+            if (4 * b + 1 < NMAX) z[4 * b + 1] = z1;
+            if (4 * b + 2 < NMAX) z[4 * b + 2] = z2;
+            if (4 * b + 3 < NMAX) z[4 * b + 3] = z3;
+        }
+    }
+}
+
+}  // namespace pmp

@@ -1,0 +1,262 @@
+"""Seeded synthetic equirectangular scenes with ground-truth depth.
+
+Scenes are ray cast in double precision: a box room (8 x 6 x 3 m) whose
+surfaces carry 4-octave value noise, optionally 20 random spheres and an open
+cylindrical tube segment of 1.2 m diameter.  Cameras are equirectangular with
+the reference's bearing convention (``PatchMatchCuda.cu:191-198``):
+b = (cos(lat) sin(lon), -sin(lat), cos(lat) cos(lon)), lon = 2 pi (j/W - 1/2),
+lat = -pi (i/H - 1/2); so camera y points down and z forward.  Poses are
+world->camera (R, t) as ``data/keyframe.cc:235-243`` returns them.
+
+Depth is the distance along the unit bearing, which is what
+1/|b.p| measures for a plane p in the densifier.
+"""
+from __future__ import annotations
+
+import dataclasses
+import numpy as np
+
+ROOM_LO = np.array([-4.0, -1.5, -3.0])
+ROOM_HI = np.array([4.0, 1.5, 3.0])
+
+
+def _split_seeds(seed: int, n: int) -> list[int]:
+    """SplitMix64 stream of n 64-bit seeds from one master seed."""
+    out = []
+    x = seed & 0xFFFFFFFFFFFFFFFF
+    for _ in range(n):
+        x = (x + 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF
+        z = x
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & 0xFFFFFFFFFFFFFFFF
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & 0xFFFFFFFFFFFFFFFF
+        out.append(z ^ (z >> 31))
+    return out
+
+
+class ValueNoise3:
+    """3-D value noise: random lattice values, quintic-faded trilinear blend."""
+
+    def __init__(self, seed: int, size: int = 256):
+        rng = np.random.default_rng(seed)
+        self.perm = rng.permutation(size).astype(np.int64)
+        self.perm = np.concatenate([self.perm, self.perm])
+        self.vals = rng.uniform(-1.0, 1.0, size)
+        self.mask = size - 1
+
+    def _hash(self, ix, iy, iz):
+        p = self.perm
+        m = self.mask
+        return p[(p[(p[ix & m] + iy) & m] + iz) & m]
+
+    def __call__(self, x, y, z):
+        ix, iy, iz = np.floor(x).astype(np.int64), np.floor(y).astype(np.int64), np.floor(z).astype(np.int64)
+        fx, fy, fz = x - ix, y - iy, z - iz
+        fade = lambda t: t * t * t * (t * (t * 6.0 - 15.0) + 10.0)
+        ux, uy, uz = fade(fx), fade(fy), fade(fz)
+        v = self.vals
+        out = 0.0
+        for dx in (0, 1):
+            wx = ux if dx else 1.0 - ux
+            for dy in (0, 1):
+                wy = uy if dy else 1.0 - uy
+                for dz in (0, 1):
+                    wz = uz if dz else 1.0 - uz
+                    out = out + wx * wy * wz * v[self._hash(ix + dx, iy + dy, iz + dz)]
+        return out
+
+
+@dataclasses.dataclass
+class Scene:
+    spheres: np.ndarray          # (S, 4): cx, cy, cz, r
+    tube: tuple | None           # (y0, z0, radius, x0, x1) axis along x
+    noise: ValueNoise3
+    base_freq: float = 1.0
+
+    def texture(self, P, obj):
+        """4-octave value noise at world points P (n, 3); obj shifts the lattice."""
+        off = (obj.astype(np.float64) * 17.31)[:, None]
+        Q = P * self.base_freq + off
+        val = np.zeros(P.shape[0])
+        amp, f = 1.0, 1.0
+        for _ in range(4):
+            val += amp * self.noise(Q[:, 0] * f, Q[:, 1] * f, Q[:, 2] * f)
+            amp *= 0.5
+            f *= 2.0
+        return np.clip(128.0 + 110.0 * val, 0.0, 255.0)
+
+    def cast(self, C, D):
+        """Nearest hit of rays C + t D (D unit, (n,3)); returns t and object ids."""
+        n = D.shape[0]
+        with np.errstate(divide="ignore", invalid="ignore"):
+            tb = np.where(D > 0, (ROOM_HI - C) / D, (ROOM_LO - C) / D)
+        tb = np.where(np.isfinite(tb) & (tb > 0), tb, np.inf)
+        face = np.argmin(tb, axis=1)
+        t = tb[np.arange(n), face]
+        obj = face * 2 + (D[np.arange(n), face] > 0)          # 0..5 walls
+        for s, (cx, cy, cz, r) in enumerate(self.spheres):
+            oc = C - np.array([cx, cy, cz])
+            b = D @ oc if oc.ndim == 1 else np.einsum("ij,ij->i", D, oc)
+            c = float(oc @ oc) - r * r if oc.ndim == 1 else np.einsum("ij,ij->i", oc, oc) - r * r
+            disc = b * b - c
+            ok = disc > 0
+            sq = np.sqrt(np.where(ok, disc, 0.0))
+            t1 = -b - sq
+            t2 = -b + sq
+            ts = np.where(t1 > 1e-9, t1, np.where(t2 > 1e-9, t2, np.inf))
+            ts = np.where(ok, ts, np.inf)
+            hit = ts < t
+            t = np.where(hit, ts, t)
+            obj = np.where(hit, 6 + s, obj)
+        if self.tube is not None:
+            y0, z0, R, x0, x1 = self.tube
+            oy = C[..., 1] - y0
+            oz = C[..., 2] - z0
+            a = D[:, 1] ** 2 + D[:, 2] ** 2
+            b = D[:, 1] * oy + D[:, 2] * oz
+            c = oy * oy + oz * oz - R * R
+            disc = b * b - a * c
+            ok = (disc > 0) & (a > 1e-12)
+            sq = np.sqrt(np.where(ok, disc, 0.0))
+            aa = np.where(a > 1e-12, a, 1.0)
+            best = np.full(n, np.inf)
+            for root in ((-b - sq) / aa, (-b + sq) / aa):
+                xh = C[..., 0] + root * D[:, 0]
+                valid = ok & (root > 1e-9) & (xh >= x0) & (xh <= x1)
+                best = np.where(valid & (root < best), root, best)
+            hit = best < t
+            t = np.where(hit, best, t)
+            obj = np.where(hit, 6 + len(self.spheres), obj)
+        return t, obj
+
+
+def bearings(W: int, H: int) -> np.ndarray:
+    """Unit bearings (H, W, 3) in the camera frame, double precision."""
+    j = np.arange(W, dtype=np.float64)
+    i = np.arange(H, dtype=np.float64)
+    lon = 2.0 * np.pi * (j / W - 0.5)
+    lat = -np.pi * (i / H - 0.5)
+    cl, sl = np.cos(lat)[:, None], np.sin(lat)[:, None]
+    return np.stack([cl * np.sin(lon)[None, :], np.broadcast_to(-sl, (H, W)), cl * np.cos(lon)[None, :]], -1)
+
+
+def yaw_pose(yaw: float, centre) -> np.ndarray:
+    """12-vector world->camera pose (R row-major, t) of a camera at centre with yaw about y."""
+    c, s = np.cos(yaw), np.sin(yaw)
+    R_wc = np.array([[c, 0.0, s], [0.0, 1.0, 0.0], [-s, 0.0, c]])
+    R_cw = R_wc.T
+    t = -R_cw @ np.asarray(centre, dtype=np.float64)
+    return np.concatenate([R_cw.ravel(), t])
+
+
+def render(scene: Scene, pose: np.ndarray, W: int, H: int):
+    """Noise-free intensity (float, H x W) and depth (H x W) of one camera."""
+    R = pose[:9].reshape(3, 3)
+    t = pose[9:]
+    C = -R.T @ t
+    B = bearings(W, H).reshape(-1, 3)
+    D = B @ R            # rows: R^T b
+    depth, obj = scene.cast(C, D)
+    P = C + depth[:, None] * D
+    img = scene.texture(P, obj)
+    return img.reshape(H, W), depth.reshape(H, W)
+
+
+def photometric(img: np.ndarray, rng: np.random.Generator) -> np.ndarray:
+    """gain U(0.8,1.2), bias U(-10,10), N(0, 2^2) noise, clamp, round -> uint8."""
+    gain = rng.uniform(0.8, 1.2)
+    bias = rng.uniform(-10.0, 10.0)
+    out = img * gain + bias + rng.normal(0.0, 2.0, img.shape)
+    return np.clip(np.rint(out), 0, 255).astype(np.uint8)
+
+
+def make_scene(seed: int, spheres: int = 0, tube: bool = False, cam_region: float = 0.8) -> Scene:
+    s_scene, s_tex = _split_seeds(seed, 2)
+    rng = np.random.default_rng(s_scene)
+    sph = []
+    while len(sph) < spheres:
+        r = rng.uniform(0.15, 0.5)
+        c = rng.uniform(ROOM_LO + r + 0.05, ROOM_HI - r - 0.05)
+        if np.hypot(c[0], c[2]) < r + cam_region + 0.6:
+            continue
+        if tube and abs(c[2] - 2.0) < r + 0.7 and -1.6 < c[0] < 0.6:
+            continue
+        sph.append([c[0], c[1], c[2], r])
+    tb = (0.5, 2.0, 0.6, -1.5, 0.5) if tube else None
+    return Scene(np.array(sph, dtype=np.float64).reshape(-1, 4), tb, ValueNoise3(s_tex))
+
+
+@dataclasses.dataclass
+class Keyframe:
+    """One reference keyframe and its neighbourhood, ready for the densifier."""
+    W: int
+    H: int
+    ref: np.ndarray               # (H, W) uint8
+    views: list                   # N x (H, W) uint8
+    ref_pose: np.ndarray          # (12,)
+    view_poses: np.ndarray        # (N, 12)
+    mask: np.ndarray              # (H, W) uint8, 0 = ignored
+    gt_depth: np.ndarray          # (H, W) float64, reference camera
+    bgr: np.ndarray               # (H, W, 3) uint8 colour thumbnail (grey replicated)
+
+
+def make_keyframe(seed: int, W: int, H: int, n_views: int, baseline=(0.1, 0.3),
+                  spheres: int = 0, tube: bool = False, mask_frac: float = 0.15) -> Keyframe:
+    """Reference camera near the room centre plus n_views neighbours at random
+    baselines/yaws (seeded)."""
+    s_scene, s_pose, s_photo = _split_seeds(seed ^ 0x5EED, 3)
+    scene = make_scene(s_scene, spheres=spheres, tube=tube)
+    rng = np.random.default_rng(s_pose)
+    c0 = np.array([rng.uniform(-0.2, 0.2), rng.uniform(-0.05, 0.05), rng.uniform(-0.2, 0.2)])
+    ref_pose = yaw_pose(rng.uniform(-np.pi, np.pi), c0)
+    poses = []
+    for _ in range(n_views):
+        az = rng.uniform(-np.pi, np.pi)
+        el = rng.uniform(-0.2, 0.2)
+        L = rng.uniform(*baseline)
+        d = np.array([np.cos(el) * np.sin(az), np.sin(el), np.cos(el) * np.cos(az)])
+        poses.append(yaw_pose(rng.uniform(-np.pi, np.pi), c0 + L * d))
+    prng = np.random.default_rng(s_photo)
+    img, gt = render(scene, ref_pose, W, H)
+    ref = photometric(img, prng)
+    views = [photometric(render(scene, p, W, H)[0], prng) for p in poses]
+    mask = np.ones((H, W), np.uint8)
+    mask[int(np.floor((1.0 - mask_frac) * H)):, :] = 0
+    bgr = np.repeat(ref[:, :, None], 3, axis=2)
+    return Keyframe(W, H, ref, views, ref_pose, np.array(poses), mask, gt, bgr)
+
+
+def config0(seed: int = 0) -> Keyframe:
+    """480x240, reference + 2 neighbours, box room, baselines 0.1-0.3 m."""
+    return make_keyframe(seed, 480, 240, 2, baseline=(0.1, 0.3))
+
+
+def config1(seed: int = 0, W: int = 1920, H: int = 960, n_views: int = 6) -> Keyframe:
+    """1920x960, reference + 6 neighbours, room + 20 spheres + tube, baselines 0.05-0.5 m."""
+    return make_keyframe(seed, W, H, n_views, baseline=(0.05, 0.5), spheres=20, tube=True)
+
+
+def trajectory(seed: int, n: int, spacing: float = 0.2):
+    """n camera poses on a seeded smooth closed spline through the room (yaw follows the path)."""
+    rng = np.random.default_rng(_split_seeds(seed ^ 0x7A7, 1)[0])
+    k = 6
+    ang = np.linspace(0, 2 * np.pi, k, endpoint=False)
+    rad = rng.uniform(0.8, 1.6, k)
+    ctrl = np.stack([rad * np.cos(ang) * 1.6, rng.uniform(-0.1, 0.1, k), rad * np.sin(ang)], 1)
+    # Catmull-Rom, densely sampled, then re-sampled by arc length
+    pts = []
+    for a in range(k):
+        p0, p1, p2, p3 = ctrl[(a - 1) % k], ctrl[a], ctrl[(a + 1) % k], ctrl[(a + 2) % k]
+        for s in np.linspace(0, 1, 200, endpoint=False):
+            pts.append(0.5 * ((2 * p1) + (-p0 + p2) * s + (2 * p0 - 5 * p1 + 4 * p2 - p3) * s * s
+                              + (-p0 + 3 * p1 - 3 * p2 + p3) * s ** 3))
+    pts = np.array(pts)
+    seg = np.linalg.norm(np.diff(pts, axis=0, append=pts[:1]), axis=1)
+    arc = np.concatenate([[0], np.cumsum(seg)])[:-1]
+    out = []
+    for m in range(n):
+        s = (m * spacing) % arc[-1]
+        q = int(np.searchsorted(arc, s, side="right") - 1)
+        nxt = pts[(q + 1) % len(pts)]
+        yaw = np.arctan2(nxt[0] - pts[q][0], nxt[2] - pts[q][2])
+        out.append(yaw_pose(yaw, pts[q]))
+    return np.array(out)

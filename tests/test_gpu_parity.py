@@ -1,0 +1,197 @@
+"""GPU parity: the HIP path through the C-ABI against the CPU oracle.
+
+Integer/index work and the canonical FP32 arithmetic are compared bit for
+bit (the oracle executes the same IEEE operation sequence); the north-star
+bar of >= 99.9 % of valid pixels within 1e-4 relative depth is asserted on
+top of exact equality statistics that are printed.
+"""
+import numpy as np
+import pytest
+
+from oracle import or_py
+from patchmatch_amd.params import (AGG_MAX, AGG_REF_FOLD, AGG_TOPK_MEAN, COVER_FULL, EDGE_CLAMP, EDGE_ZERO,
+                                   config0_params, config1_params, default_params)
+
+from conftest import random_planes
+
+pytestmark = pytest.mark.gpu
+
+
+def _same_bits(a, b):
+    a = np.ascontiguousarray(a, np.float32).view(np.uint32)
+    b = np.ascontiguousarray(b, np.float32).view(np.uint32)
+    return a == b
+
+
+def _report(name, a, b):
+    eq = _same_bits(a, b)
+    print(f"[{name}] bit-equal {eq.mean() * 100:.4f}% of {eq.size}", flush=True)
+    return eq
+
+
+MATH_CASES = {
+    "rcp": lambda r: np.concatenate([r.uniform(-1e4, 1e4, 200000), r.uniform(-1, 1, 100000),
+                                     [0.0, -0.0, 1e-31, -1e-31, 1e31, np.inf, -np.inf, np.nan, 1e-6]]),
+    "rsqrt": lambda r: np.concatenate([10 ** r.uniform(-12, 12, 300000), [1e-31, 1e31, 1.0, 4.0]]),
+    "sqrt": lambda r: np.concatenate([10 ** r.uniform(-12, 12, 300000), [0.0, 1e-31, 1e31, -1.0, np.nan]]),
+    "asin": lambda r: np.concatenate([r.uniform(-1, 1, 300000), [-1.0, 1.0, 0.5, -0.5, 0.0, 2.0, np.nan]]),
+    "acos": lambda r: np.concatenate([r.uniform(-1, 1, 300000), [-1.0, 1.0, 0.5, -0.5, 0.0]]),
+    "exp": lambda r: np.concatenate([r.uniform(-90, 90, 300000), r.uniform(-0.2, 0.2, 100000),
+                                     [0.0, -87.3, 88.0, -np.inf, np.nan]]),
+    "log": lambda r: ((r.integers(0, 1 << 24, 300000) + 1) * 2.0 ** -24),
+    "sin2pi": lambda r: (r.integers(0, 1 << 24, 300000) * 2.0 ** -24),
+    "cos2pi": lambda r: (r.integers(0, 1 << 24, 300000) * 2.0 ** -24),
+}
+
+
+@pytest.mark.parametrize("name", sorted(MATH_CASES) + ["atan2"])
+def test_math_bitwise(gpu_ctx, name):
+    rng = np.random.default_rng(11)
+    if name == "atan2":
+        a = np.concatenate([rng.normal(0, 10, 300000), [0.0, 0.0, -1.0, 1e-35, 5.0]]).astype(np.float32)
+        b = np.concatenate([rng.normal(0, 10, 300000), [0.0, -1.0, 0.0, 1e-35, -5.0]]).astype(np.float32)
+    else:
+        a = MATH_CASES[name](rng).astype(np.float32)
+        b = None
+    g = gpu_ctx.eval_math(name, a, b)
+    o = or_py.eval_math(name, a, b)
+    eq = _same_bits(g, o) | (np.isnan(g) & np.isnan(o))
+    assert eq.all(), f"{name}: {np.count_nonzero(~eq)} mismatches, e.g. x={a[~eq][:4]} gpu={g[~eq][:4]} cpu={o[~eq][:4]}"
+
+
+def test_refine_normals_bitwise(gpu_ctx, kf_c0):
+    ok = or_py.Keyframe.from_synth(kf_c0)
+    p = config0_params(seed=0x1234567890ABCDEF)
+    for (i, j, it) in [(0, 0, 0), (10, 20, 3), (239, 479, 7), (100, 101, 1)]:
+        g = gpu_ctx.refine_normals(p, kf_c0.W, i, j, it, 18)
+        o = or_py.refine_normals(ok, p, i, j, it, 18)
+        assert _same_bits(g, o).all(), (i, j, it, g, o)
+
+
+def _load(ctx, kf):
+    ctx.load_keyframe(kf.ref, kf.views, kf.ref_pose, kf.view_poses)
+
+
+@pytest.mark.parametrize("pname", ["c0", "c0_fold_ref_init", "c0_max_clamp", "c0_zero_edge"])
+def test_init_bitwise(gpu_ctx, kf_c0, pname):
+    p = {"c0": config0_params(),
+         "c0_fold_ref_init": config0_params(init_mode=0, agg_mode=AGG_REF_FOLD, max_depth=8.0, seed=7),
+         "c0_max_clamp": config0_params(agg_mode=AGG_MAX, edge_mode=EDGE_CLAMP, seed=3),
+         "c0_zero_edge": config0_params(edge_mode=EDGE_ZERO, coverage_mode=COVER_FULL, seed=5)}[pname]
+    _load(gpu_ctx, kf_c0)
+    print(f"case {pname}", flush=True)
+    gpu_ctx.random_initialization(p)
+    gp, gd, gs = gpu_ctx.get_maps(0)
+    op, od, os_ = or_py.init(or_py.Keyframe.from_synth(kf_c0), p)
+    assert _report("plane", gp, op).all()
+    assert _report("depth", gd, od).all()
+    assert _report("score", gs, os_).all()
+    ip, idp, isc = gpu_ctx.get_maps(1)
+    assert _same_bits(ip, gp).all() and _same_bits(idp, gd).all() and _same_bits(isc, gs).all()
+
+
+@pytest.mark.parametrize("agg", [AGG_REF_FOLD, AGG_MAX, AGG_TOPK_MEAN])
+def test_score_field_bitwise(gpu_ctx, kf_small6, agg):
+    kf = kf_small6
+    p = config1_params(agg_mode=agg, coverage_mode=COVER_FULL)
+    _load(gpu_ctx, kf)
+    planes = random_planes(kf, 5)
+    gpu_ctx.set_maps(planes, np.ones((kf.H, kf.W), np.float32), np.zeros((kf.H, kf.W), np.float32))
+    gpu_ctx.score_plane_depth(p)
+    _, _, gs = gpu_ctx.get_maps(0)
+    os_ = or_py.score_field(or_py.Keyframe.from_synth(kf), p, planes)
+    ok = ~np.isnan(os_)
+    assert _report("score", gs[ok], os_[ok]).all()
+
+
+def test_propagate_c0(gpu_ctx, kf_c0):
+    """Config 0: init + 4 red-black iterations, compared map by map."""
+    p = config0_params()
+    _load(gpu_ctx, kf_c0)
+    gpu_ctx.random_initialization(p)
+    ok = or_py.Keyframe.from_synth(kf_c0)
+    op, od, os_ = or_py.init(ok, p)
+    for it in range(p.iterations):
+        gpu_ctx.red_black_pass(p, it, it + 1)
+        op, od, os_, _ = or_py.propagate(ok, p, op, od, os_, it, it + 1)
+        gp, gd, gs = gpu_ctx.get_maps(0)
+        eqd = _report(f"it{it} depth", gd, od)
+        _report(f"it{it} plane", gp, op)
+        _report(f"it{it} score", gs, os_)
+    valid = od > 0
+    rel = np.abs(gd.astype(np.float64) - od) / np.maximum(od, 1e-30)
+    frac = np.mean(rel[valid] <= 1e-4)
+    print(f"depth within 1e-4 relative: {frac * 100:.4f}%", flush=True)
+    assert frac >= 0.999
+    # checkerboard/coverage/mask bit-exact: the set of pixels that changed is identical
+    assert np.array_equal(np.isnan(gs), np.isnan(os_))
+
+
+def test_densify_c0_final(gpu_ctx, kf_c0):
+    p = config0_params()
+    _load(gpu_ctx, kf_c0)
+    gpu_ctx.densify(p, kf_c0.mask)
+    gp, gd, gs, gn = gpu_ctx.get_maps(2)
+    out = or_py.densify(or_py.Keyframe.from_synth(kf_c0), p, kf_c0.mask)
+    assert np.array_equal(gn, out["nghbr"])
+    assert np.array_equal(gd == 0, out["depth"] == 0)
+    eq = _report("final depth", gd, out["depth"])
+    valid = out["depth"] > 0
+    rel = np.abs(gd.astype(np.float64) - out["depth"]) / np.maximum(out["depth"], 1e-30)
+    assert np.mean(rel[valid] <= 1e-4) >= 0.999
+    _report("final score", gs, out["score"])
+    _report("final plane", gp[..., :3], out["plane"])
+    assert eq.mean() >= 0.999
+
+
+def test_clean_bitwise(gpu_ctx, kf_c0):
+    from patchmatch_amd import synth
+    rng = np.random.default_rng(2)
+    traj = synth.trajectory(1, 5, 0.15)
+    scene = synth.make_scene(42)
+    depths = []
+    for pose in traj:
+        _, d = synth.render(scene, pose, 240, 120)
+        d = d.astype(np.float32) * rng.choice([1.0, 1.0, 1.0, 1.2], d.shape).astype(np.float32)
+        d[rng.random(d.shape) < 0.05] = 0.0
+        depths.append(d)
+    g = gpu_ctx.clean(depths, traj, 0.03, 3)
+    o = or_py.clean(depths, traj, 0.03, 3)
+    print(f"kept {np.count_nonzero(o)} of {o.size}")
+    assert _same_bits(g, o).all()
+
+
+def test_prune_sequence_bitwise(gpu_ctx):
+    """Five queued maps, three prunes with pops: pruned maps, modified later maps
+    and emitted points equal the oracle's raster-order loop exactly."""
+    from patchmatch_amd import synth
+    rng = np.random.default_rng(9)
+    traj = synth.trajectory(2, 8, 0.1)
+    scene = synth.make_scene(43)
+    maps, bgrs = [], []
+    for pose in traj:
+        img, d = synth.render(scene, pose, 200, 100)
+        d = d.astype(np.float32) * rng.choice([1.0, 1.0, 1.005, 0.97], d.shape).astype(np.float32)
+        d[rng.random(d.shape) < 0.03] = 0.0
+        maps.append(d)
+        bgrs.append(np.repeat(np.clip(img, 0, 255).astype(np.uint8)[..., None], 3, 2))
+    gpu_ctx.pruner_reset()
+    cpu_maps = [m.copy() for m in maps]
+    for k in range(5):
+        gpu_ctx.pruner_add_view(maps[k], traj[k], bgrs[k])
+    queue = list(range(5))
+    nxt = 5
+    for step in range(3):
+        gpr, gpts, gcols, gkps = gpu_ctx.pruner_prune(0.03, maps[0].shape)
+        opr, opts, ocols, okps = or_py.prune([cpu_maps[q] for q in queue], traj[queue], bgrs[queue[0]], 0.03)
+        print(f"step {step}: {len(opts)} points", flush=True)
+        assert _same_bits(gpr, opr).all()
+        assert np.array_equal(gkps, okps) and np.array_equal(gcols, ocols)
+        assert np.array_equal(gpts.view(np.uint64), opts.view(np.uint64))
+        for idx, q in enumerate(queue):
+            assert _same_bits(gpu_ctx.pruner_get_map(idx, maps[0].shape), cpu_maps[q]).all()
+        gpu_ctx.pruner_pop_head()
+        queue.pop(0)
+        gpu_ctx.pruner_add_view(maps[nxt], traj[nxt], bgrs[nxt])
+        queue.append(nxt)
+        nxt += 1
