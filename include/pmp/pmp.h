@@ -138,9 +138,14 @@ int pmp_score_plane_depth(pmp_ctx* ctx, const pmp_params* params);
  * [it_begin, it_end), each a pass over (i+j) even pixels then (i+j) odd. */
 int pmp_red_black_pass(pmp_ctx* ctx, const pmp_params* params, int32_t it_begin, int32_t it_end);
 
+/* Select a pooled image (same size as the reference) as the device-resident
+ * mask used by pmp_finalize/pmp_densify when their mask argument is NULL;
+ * mask_id < 0 clears it (then NULL means every pixel is valid). */
+int pmp_set_mask(pmp_ctx* ctx, int64_t mask_id);
+
 /* Copy-back window, low-texture gate and PostProcess of ComputePatchMatch
  * (depthmap.cc:673-687, :731-767, :1169-1208).  mask: host H*W bytes,
- * 0 = ignored pixel; NULL = all valid. */
+ * 0 = ignored pixel; NULL = the pmp_set_mask image, or all valid. */
 int pmp_finalize(pmp_ctx* ctx, const pmp_params* params, const uint8_t* mask);
 
 /* The whole estimator: init, params->iterations red-black iterations,
@@ -162,6 +167,21 @@ int pmp_get_maps(pmp_ctx* ctx, int32_t which, float* plane4, float* depth, float
 /* Counters of the last pmp_red_black_pass/pmp_densify: hypotheses scored
  * (evals) and scored+pre-rejected (candidates), summed over pixels. */
 int pmp_eval_counts(pmp_ctx* ctx, int64_t* evals, int64_t* candidates);
+
+/* Device time per stage, from HIP events recorded on the context's stream
+ * around every launch while timing is enabled (off by default).  Replaces the
+ * cudaEvent + printf timing of every reference wrapper
+ * (PatchMatchCuda.cu:421-441, :570-601).  pmp_get_timing waits for the
+ * stream, reports the launches recorded since the previous call, and resets. */
+typedef struct pmp_timing {
+    double init_ms;           /* random initialisation + initial score */
+    double propagate_ms;      /* sum over red-black pass launches */
+    double finalize_ms;       /* copy-back, gate, PostProcess */
+    int32_t propagate_launches;
+    int32_t densify_calls;
+} pmp_timing;
+int pmp_set_timing(pmp_ctx* ctx, int32_t enable);
+int pmp_get_timing(pmp_ctx* ctx, pmp_timing* out);
 
 /* ---- geometric consistency (DepthmapCleaner, depthmap.cc:1320-1377) ----- */
 

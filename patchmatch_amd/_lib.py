@@ -43,7 +43,13 @@ EXPORTS = [
     "pmp_densify", "pmp_set_maps", "pmp_get_maps", "pmp_eval_counts", "pmp_clean",
     "pmp_pruner_add_view", "pmp_pruner_prune", "pmp_pruner_pop_head", "pmp_pruner_reset",
     "pmp_pruner_size", "pmp_pruner_get_map", "pmp_debug_eval_math", "pmp_debug_refine_normals",
+    "pmp_set_timing", "pmp_get_timing", "pmp_set_mask",
 ]
+
+
+class Timing(ctypes.Structure):
+    _fields_ = [("init_ms", ctypes.c_double), ("propagate_ms", ctypes.c_double), ("finalize_ms", ctypes.c_double),
+                ("propagate_launches", ctypes.c_int32), ("densify_calls", ctypes.c_int32)]
 
 _lib = None
 
@@ -89,6 +95,9 @@ def load():
         "pmp_pruner_get_map": ([_vp, i32, _f32p], c_int),
         "pmp_debug_eval_math": ([_vp, i32, i64, _f32p, _f32p, _f32p], c_int),
         "pmp_debug_refine_normals": ([_vp, P(Params), i32, i32, i32, i32, i32, _f32p], c_int),
+        "pmp_set_timing": ([_vp, i32], c_int),
+        "pmp_get_timing": ([_vp, P(Timing)], c_int),
+        "pmp_set_mask": ([_vp, i64], c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -185,6 +194,17 @@ class Context:
     def densify(self, p: Params, mask=None):
         m = None if mask is None else np.ascontiguousarray(mask, dtype=np.uint8)
         self._chk(self.L.pmp_densify(self.h, ctypes.byref(p), _p(m, _u8p)), "pmp_densify")
+
+    def set_mask(self, mask_id: int):
+        self._chk(self.L.pmp_set_mask(self.h, mask_id), "pmp_set_mask")
+
+    def set_timing(self, enable: bool):
+        self._chk(self.L.pmp_set_timing(self.h, 1 if enable else 0), "pmp_set_timing")
+
+    def get_timing(self) -> dict:
+        t = Timing()
+        self._chk(self.L.pmp_get_timing(self.h, ctypes.byref(t)), "pmp_get_timing")
+        return {f: getattr(t, f) for f, _ in Timing._fields_}
 
     def synchronize(self):
         self._chk(self.L.pmp_synchronize(self.h), "pmp_synchronize")

@@ -120,10 +120,28 @@ int ensure_tables(pmp_ctx* ctx, int W, int H, float sigma_color) {
     return PMP_OK;
 }
 
+int timing_begin(pmp_ctx* ctx, int stage) {
+    if (!ctx->timing) return -1;
+    const size_t k = ctx->events_used;
+    while (ctx->events.size() < 2 * (k + 1)) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return -1;
+        ctx->events.push_back(e);
+    }
+    if (ctx->event_stage.size() < k + 1) ctx->event_stage.resize(k + 1);
+    ctx->event_stage[k] = stage;
+    if (hipEventRecord(ctx->events[2 * k], ctx->stream) != hipSuccess) return -1;
+    ctx->events_used = k + 1;
+    return (int)k;
+}
+
+void timing_end(pmp_ctx* ctx, int pair) {
+    if (pair < 0) return;
+    (void)hipEventRecord(ctx->events[2 * (size_t)pair + 1], ctx->stream);
+}
+
 static void free_image(Image& im) {
     if (im.gray) (void)hipFree(im.gray);
-    if (im.quad_wrap) (void)hipFree(im.quad_wrap);
-    if (im.quad_clamp) (void)hipFree(im.quad_clamp);
     im = Image();
 }
 
@@ -159,13 +177,15 @@ void pmp_destroy(pmp_ctx* ctx) {
                     ctx->init.score, ctx->out_plane, ctx->out_depth, ctx->out_score, ctx->out_nghbr,
                     ctx->tmp0, ctx->tmp1, ctx->mask, ctx->rowtab, ctx->coltab, ctx->wc, ctx->counters,
                     ctx->clean_maps, ctx->clean_poses, ctx->prune_z, ctx->prune_alive, ctx->prune_out,
-                    ctx->prune_pts, ctx->prune_cols, ctx->prune_kps, ctx->prune_blk, ctx->prune_poses, ctx->dbg};
+                    ctx->prune_pts, ctx->prune_cols, ctx->prune_kps, ctx->prune_blk, ctx->prune_poses, ctx->dbg,
+                    ctx->views_q};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     for (auto& v : ctx->pruner) {
         if (v.depth) (void)hipFree(v.depth);
         if (v.bgr) (void)hipFree(v.bgr);
     }
+    for (hipEvent_t e : ctx->events) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -211,6 +231,33 @@ int pmp_synchronize(pmp_ctx* ctx) {
     return PMP_OK;
 }
 
+int pmp_set_timing(pmp_ctx* ctx, int32_t enable) {
+    if (!ctx) return PMP_EINVAL;
+    ctx->timing = enable != 0;
+    ctx->events_used = 0;
+    ctx->densify_calls = 0;
+    return PMP_OK;
+}
+
+int pmp_get_timing(pmp_ctx* ctx, pmp_timing* out) {
+    if (!ctx || !out) return PMP_EINVAL;
+    memset(out, 0, sizeof(*out));
+    PMP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    for (size_t k = 0; k < ctx->events_used; ++k) {
+        float ms = 0.0f;
+        PMP_HIP(ctx, hipEventElapsedTime(&ms, ctx->events[2 * k], ctx->events[2 * k + 1]));
+        switch (ctx->event_stage[k]) {
+            case kStageInit: out->init_ms += ms; break;
+            case kStagePropagate: out->propagate_ms += ms; out->propagate_launches++; break;
+            default: out->finalize_ms += ms; break;
+        }
+    }
+    out->densify_calls = ctx->densify_calls;
+    ctx->events_used = 0;
+    ctx->densify_calls = 0;
+    return PMP_OK;
+}
+
 int pmp_image_put(pmp_ctx* ctx, uint64_t id, const uint8_t* pixels, int32_t width, int32_t height, int64_t pitch,
                   int32_t on_device) {
     if (!ctx || !pixels || width <= 0 || height <= 0 || pitch < width) return PMP_EINVAL;
@@ -224,6 +271,7 @@ int pmp_image_put(pmp_ctx* ctx, uint64_t id, const uint8_t* pixels, int32_t widt
     Image im;
     im.W = width;
     im.H = height;
+    im.generation = ctx->next_generation++;
     const size_t bytes = (size_t)width * height;
     PMP_HIP(ctx, hipMalloc(&im.gray, bytes));
     im.bytes = bytes;

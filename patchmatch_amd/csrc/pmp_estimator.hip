@@ -54,7 +54,7 @@ __global__ void k_pack(const uint8_t* __restrict__ g, uint32_t* __restrict__ q, 
 }
 
 // ---------------------------------------------------------------- init
-template <int NV>
+template <int NV, bool kZero>
 __global__ __launch_bounds__(256) void k_init(KfArgs a, float4* __restrict__ plane, float* __restrict__ depth,
                                               float* __restrict__ score, float4* __restrict__ iplane,
                                               float* __restrict__ idepth, float* __restrict__ iscore) {
@@ -68,7 +68,7 @@ __global__ __launch_bounds__(256) void k_init(KfArgs a, float4* __restrict__ pla
     float sc = __uint_as_float(0xFFFFFFFFu);   // cudaMemset(score, 0xFF) (depthmap.cc:657)
     if (in_score_grid(a, i, j)) {
         RefStat rs = ref_stats(a, sWc, i, j);
-        sc = score_plane<NV>(a, sWc, rs, i, j, p0, p1, p2);
+        sc = score_plane<NV, kZero>(a, sWc, rs, i, j, p0, p1, p2);
     }
     const size_t k = (size_t)i * a.W + j;
     const float4 pl = make_float4(p0, p1, p2, 0.0f);
@@ -77,7 +77,7 @@ __global__ __launch_bounds__(256) void k_init(KfArgs a, float4* __restrict__ pla
 }
 
 // ---------------------------------------------------------------- score field
-template <int NV>
+template <int NV, bool kZero>
 __global__ __launch_bounds__(256) void k_score(KfArgs a, const float4* __restrict__ plane,
                                                float* __restrict__ score) {
     __shared__ float sWc[256];
@@ -88,14 +88,14 @@ __global__ __launch_bounds__(256) void k_score(KfArgs a, const float4* __restric
     const size_t k = (size_t)i * a.W + j;
     RefStat rs = ref_stats(a, sWc, i, j);
     const float4 p = plane[k];
-    score[k] = score_plane<NV>(a, sWc, rs, i, j, p.x, p.y, p.z);
+    score[k] = score_plane<NV, kZero>(a, sWc, rs, i, j, p.x, p.y, p.z);
 }
 
 // ---------------------------------------------------------------- red-black pass
 // Thread (jj, i) updates pixel j = 2 jj + ((i + colour) & 1): every pixel of
 // colour (i+j)&1 == colour once.  Candidates are at odd distance (1 or 3), so
 // they all have the other colour and are not written by this launch.
-template <int NV>
+template <int NV, bool kZero>
 __global__ __launch_bounds__(256) void k_propagate(KfArgs a, float4* __restrict__ plane,
                                                    float* __restrict__ depth, float* __restrict__ score, int it,
                                                    int colour, unsigned long long* __restrict__ counters) {
@@ -139,7 +139,7 @@ __global__ __launch_bounds__(256) void k_propagate(KfArgs a, float4* __restrict_
             const float f = (1.0f - fd) * (1.0f - fp);
             // |score| <= 1: the penalised score cannot exceed f
             if (!(f > sc)) continue;
-            float sh = score_plane<NV>(a, sWc, rs, i, j, pn.x, pn.y, pn.z);
+            float sh = score_plane<NV, kZero>(a, sWc, rs, i, j, pn.x, pn.y, pn.z);
             ++n_eval;
             sh = sh * f;
             if (sh > sc) { h0 = pn.x; h1 = pn.y; h2 = pn.z; sc = sh; }
@@ -162,7 +162,7 @@ __global__ __launch_bounds__(256) void k_propagate(KfArgs a, float4* __restrict_
             const float nz = fma_(a.nr[t], rc, c2);
             float q0, q1, q2;
             plane_from_depth_normal(bx, by, bz, dh, nx, ny, nz, q0, q1, q2);
-            const float sh = score_plane<NV>(a, sWc, rs, i, j, q0, q1, q2);
+            const float sh = score_plane<NV, kZero>(a, sWc, rs, i, j, q0, q1, q2);
             ++n_eval;
             ++n_cand;
             if (sh > sc) { c0 = q0; c1 = q1; c2 = q2; sc = sh; dcur = dh; }
@@ -306,26 +306,42 @@ static int fill_args(pmp_ctx* ctx, const pmp_params* p, KfArgs& a) {
     int rc = ensure_tables(ctx, a.W, a.H, p->sigma_color);
     if (rc) return rc;
     a.ref = ctx->images[ctx->ref_id].gray;
+    // neighbours packed contiguously (view v at v*W*H) for the current edge mode
+    std::vector<uint64_t> key;
+    key.push_back(p->edge_mode == PMP_EDGE_WRAP ? 1 : 0);
     for (int v = 0; v < a.N; ++v) {
-        Image& im = ctx->images[ctx->view_ids[v]];
-        uint32_t*& q = (p->edge_mode == PMP_EDGE_WRAP) ? im.quad_wrap : im.quad_clamp;
-        if (!q) {
-            size_t bytes = (size_t)im.W * im.H * 4;
-            PMP_HIP(ctx, hipMalloc(&q, bytes));
-            im.bytes += bytes;
+        key.push_back(ctx->view_ids[v]);
+        key.push_back(ctx->images[ctx->view_ids[v]].generation);
+    }
+    const size_t P = (size_t)a.W * a.H;
+    if (key != ctx->packed_key) {
+        if (ctx->views_q_cap < (size_t)a.N * P) {
+            PMP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+            if (ctx->views_q) (void)hipFree(ctx->views_q);
+            ctx->views_q = nullptr;
+            ctx->views_q_cap = 0;
+            PMP_HIP(ctx, hipMalloc(&ctx->views_q, (size_t)a.N * P * sizeof(uint32_t)));
+            ctx->views_q_cap = (size_t)a.N * P;
+        }
+        for (int v = 0; v < a.N; ++v) {
+            const Image& im = ctx->images[ctx->view_ids[v]];
             dim3 blk(256), grd((im.W + 255) / 256, im.H);
-            hipLaunchKernelGGL(k_pack, grd, blk, 0, ctx->stream, im.gray, q, im.W, im.H,
+            hipLaunchKernelGGL(k_pack, grd, blk, 0, ctx->stream, im.gray, ctx->views_q + v * P, im.W, im.H,
                                p->edge_mode == PMP_EDGE_WRAP ? 1 : 0);
             PMP_HIP(ctx, hipGetLastError());
         }
-        a.quads[v] = q;
+        ctx->packed_key = key;
     }
+    a.quads = ctx->views_q;
+    a.quads_bytes = (uint32_t)((size_t)a.N * P * sizeof(uint32_t));
     a.rowtab = ctx->rowtab; a.coltab = ctx->coltab; a.Wc = ctx->wc;
     derive_poses(ctx->ref_pose, a.N, ctx->view_poses.data(), a.M, a.c);
     a.ku = (float)((double)a.W / (2.0 * M_PI));
     a.hu = (float)(0.5 * a.W);
     a.kv = (float)((double)a.H / M_PI);
     a.hv = (float)(0.5 * a.H);
+    a.umax = (p->edge_mode == PMP_EDGE_WRAP) ? (float)a.W : (float)(a.W - 1);
+    a.vmax = (float)(a.H - 1);
     a.var_floor = p->var_floor; a.fold_agree = p->fold_agree;
     a.depth_penalty = p->depth_penalty; a.plane_penalty = p->plane_penalty;
     float dr = p->refine_depth_range, nr = p->refine_normal_range;
@@ -341,25 +357,35 @@ static int fill_args(pmp_ctx* ctx, const pmp_params* p, KfArgs& a) {
     return PMP_OK;
 }
 
-#define PMP_DISPATCH_NV(N, KERNEL, GRID, BLOCK, ...)                                          \
-    switch (N) {                                                                             \
-        case 1: hipLaunchKernelGGL(KERNEL<1>, GRID, BLOCK, 0, ctx->stream, __VA_ARGS__); break;   \
-        case 2: hipLaunchKernelGGL(KERNEL<2>, GRID, BLOCK, 0, ctx->stream, __VA_ARGS__); break;   \
-        case 3: hipLaunchKernelGGL(KERNEL<3>, GRID, BLOCK, 0, ctx->stream, __VA_ARGS__); break;   \
-        case 4: hipLaunchKernelGGL(KERNEL<4>, GRID, BLOCK, 0, ctx->stream, __VA_ARGS__); break;   \
-        case 5: hipLaunchKernelGGL(KERNEL<5>, GRID, BLOCK, 0, ctx->stream, __VA_ARGS__); break;   \
-        case 6: hipLaunchKernelGGL(KERNEL<6>, GRID, BLOCK, 0, ctx->stream, __VA_ARGS__); break;   \
-        case 7: hipLaunchKernelGGL(KERNEL<7>, GRID, BLOCK, 0, ctx->stream, __VA_ARGS__); break;   \
-        case 8: hipLaunchKernelGGL(KERNEL<8>, GRID, BLOCK, 0, ctx->stream, __VA_ARGS__); break;   \
-        case 9: hipLaunchKernelGGL(KERNEL<9>, GRID, BLOCK, 0, ctx->stream, __VA_ARGS__); break;   \
-        case 10: hipLaunchKernelGGL(KERNEL<10>, GRID, BLOCK, 0, ctx->stream, __VA_ARGS__); break; \
-        default: return set_error(ctx, PMP_EINVAL, "n_views must be in [1, 10]");            \
+#define PMP_LAUNCH_NV(N, Z, KERNEL, GRID, BLOCK, ...)                                                  \
+    switch (N) {                                                                                      \
+        case 1: hipLaunchKernelGGL((KERNEL<1, Z>), GRID, BLOCK, 0, ctx->stream, __VA_ARGS__); break;   \
+        case 2: hipLaunchKernelGGL((KERNEL<2, Z>), GRID, BLOCK, 0, ctx->stream, __VA_ARGS__); break;   \
+        case 3: hipLaunchKernelGGL((KERNEL<3, Z>), GRID, BLOCK, 0, ctx->stream, __VA_ARGS__); break;   \
+        case 4: hipLaunchKernelGGL((KERNEL<4, Z>), GRID, BLOCK, 0, ctx->stream, __VA_ARGS__); break;   \
+        case 5: hipLaunchKernelGGL((KERNEL<5, Z>), GRID, BLOCK, 0, ctx->stream, __VA_ARGS__); break;   \
+        case 6: hipLaunchKernelGGL((KERNEL<6, Z>), GRID, BLOCK, 0, ctx->stream, __VA_ARGS__); break;   \
+        case 7: hipLaunchKernelGGL((KERNEL<7, Z>), GRID, BLOCK, 0, ctx->stream, __VA_ARGS__); break;   \
+        case 8: hipLaunchKernelGGL((KERNEL<8, Z>), GRID, BLOCK, 0, ctx->stream, __VA_ARGS__); break;   \
+        case 9: hipLaunchKernelGGL((KERNEL<9, Z>), GRID, BLOCK, 0, ctx->stream, __VA_ARGS__); break;   \
+        case 10: hipLaunchKernelGGL((KERNEL<10, Z>), GRID, BLOCK, 0, ctx->stream, __VA_ARGS__); break; \
+        default: return set_error(ctx, PMP_EINVAL, "n_views must be in [1, 10]");                     \
+    }
+
+// dispatch on the view count and on zero-outside sampling (PMP_EDGE_ZERO)
+#define PMP_DISPATCH_NV(N, KERNEL, GRID, BLOCK, ...)                      \
+    if (a.edge_mode == PMP_EDGE_ZERO) {                                  \
+        PMP_LAUNCH_NV(N, true, KERNEL, GRID, BLOCK, __VA_ARGS__)         \
+    } else {                                                             \
+        PMP_LAUNCH_NV(N, false, KERNEL, GRID, BLOCK, __VA_ARGS__)        \
     }
 
 static int launch_init(pmp_ctx* ctx, const KfArgs& a) {
     dim3 blk(kBlockX, kBlockY), grd((a.W + kBlockX - 1) / kBlockX, (a.H + kBlockY - 1) / kBlockY);
+    const int tp = timing_begin(ctx, kStageInit);
     PMP_DISPATCH_NV(a.N, k_init, grd, blk, a, ctx->work.plane, ctx->work.depth, ctx->work.score, ctx->init.plane,
                     ctx->init.depth, ctx->init.score);
+    timing_end(ctx, tp);
     PMP_HIP(ctx, hipGetLastError());
     return PMP_OK;
 }
@@ -377,8 +403,10 @@ static int launch_passes(pmp_ctx* ctx, const KfArgs& a, int it_begin, int it_end
     dim3 blk(kBlockX, kBlockY), grd((half_w + kBlockX - 1) / kBlockX, (a.H + kBlockY - 1) / kBlockY);
     for (int it = it_begin; it < it_end; ++it)
         for (int colour = 0; colour < 2; ++colour) {
+            const int tp = timing_begin(ctx, kStagePropagate);
             PMP_DISPATCH_NV(a.N, k_propagate, grd, blk, a, ctx->work.plane, ctx->work.depth, ctx->work.score, it,
                             colour, ctx->counters);
+            timing_end(ctx, tp);
             PMP_HIP(ctx, hipGetLastError());
         }
     return PMP_OK;
@@ -390,8 +418,15 @@ static int launch_finalize(pmp_ctx* ctx, const KfArgs& a, const pmp_params* p, c
     if (mask_host) {
         PMP_HIP(ctx, hipMemcpyAsync(ctx->mask, mask_host, P, hipMemcpyHostToDevice, ctx->stream));
         dmask = ctx->mask;
+    } else if (ctx->mask_id >= 0) {
+        auto it = ctx->images.find((uint64_t)ctx->mask_id);
+        if (it == ctx->images.end()) return set_error(ctx, PMP_ENOTFOUND, "mask image was dropped");
+        if (it->second.W != a.W || it->second.H != a.H)
+            return set_error(ctx, PMP_EINVAL, "mask image size differs from the reference");
+        dmask = it->second.gray;
     }
     dim3 blk(256), grd((a.W + 255) / 256, a.H);
+    const int tp = timing_begin(ctx, kStageFinalize);
     hipLaunchKernelGGL(k_copyback, grd, blk, 0, ctx->stream, a, dmask, ctx->init.plane, ctx->init.depth,
                        ctx->init.score, ctx->work.plane, ctx->work.depth, ctx->work.score, ctx->out_plane,
                        ctx->out_depth, ctx->out_score, ctx->out_nghbr, p->min_patch_sd * p->min_patch_sd,
@@ -405,6 +440,7 @@ static int launch_finalize(pmp_ctx* ctx, const KfArgs& a, const pmp_params* p, c
                            p->min_score);
         PMP_HIP(ctx, hipGetLastError());
     }
+    timing_end(ctx, tp);
     return PMP_OK;
 }
 
@@ -440,6 +476,14 @@ int pmp_set_views(pmp_ctx* ctx, uint64_t ref_id, const pmp_pose* ref_pose, int32
     }
     ctx->W = W; ctx->H = H;
     ctx->views_set = true;
+    return PMP_OK;
+}
+
+int pmp_set_mask(pmp_ctx* ctx, int64_t mask_id) {
+    if (!ctx) return PMP_EINVAL;
+    if (mask_id >= 0 && ctx->images.find((uint64_t)mask_id) == ctx->images.end())
+        return set_error(ctx, PMP_ENOTFOUND, "unknown mask image id");
+    ctx->mask_id = mask_id < 0 ? -1 : mask_id;
     return PMP_OK;
 }
 
@@ -482,7 +526,9 @@ int pmp_densify(pmp_ctx* ctx, const pmp_params* params, const uint8_t* mask) {
     if (rc) return rc;
     if ((rc = launch_init(ctx, a))) return rc;
     if ((rc = launch_passes(ctx, a, 0, params->iterations))) return rc;
-    return launch_finalize(ctx, a, params, mask);
+    if ((rc = launch_finalize(ctx, a, params, mask))) return rc;
+    ctx->densify_calls++;
+    return PMP_OK;
 }
 
 int pmp_set_maps(pmp_ctx* ctx, const float* plane4, const float* depth, const float* score) {

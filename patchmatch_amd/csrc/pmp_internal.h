@@ -21,7 +21,7 @@ struct KfArgs {
     int W, H, N, half, stride, T;
     int agg_mode, topk, edge_mode, coverage, init_mode, n_refine;
     const uint8_t* ref;                   // H*W
-    const uint32_t* quads[kMaxViews];     // packed 2x2 neighbourhoods, H*W each
+    const uint32_t* quads;                // N*H*W packed 2x2 neighbourhoods, view v at v*H*W
     const float2* rowtab;                 // H: (cos lat, sin lat)
     const float2* coltab;                 // W: (sin lon, cos lon)
     const float* Wc;                      // 256: exp(-a^2 / (2 sigma_color^2))
@@ -29,6 +29,8 @@ struct KfArgs {
     float M[kMaxViews][9];                // R_n R_ref^T
     float c[kMaxViews][3];                // t_n - M t_ref
     float ku, hu, kv, hv;                 // u = lon*ku + hu, v = asin*kv + hv
+    float umax, vmax;                     // sample clamps: W (wrap) or W-1, and H-1
+    uint32_t quads_bytes;                 // size of the packed-view buffer
     float var_floor, fold_agree, inv_k;
     float depth_penalty, plane_penalty;
     float dr[kMaxRefine], nr[kMaxRefine];
@@ -52,9 +54,8 @@ struct DevBuf {
 struct Image {
     int W = 0, H = 0;
     uint8_t* gray = nullptr;          // H*W
-    uint32_t* quad_wrap = nullptr;    // packed for longitude wrap
-    uint32_t* quad_clamp = nullptr;   // packed with clamped right edge
     size_t bytes = 0;
+    uint64_t generation = 0;          // changes whenever the id is re-uploaded
 };
 
 struct PrunerView {
@@ -78,6 +79,7 @@ struct pmp_ctx {
     uint64_t ref_id = 0;
     std::vector<uint64_t> view_ids;
     double ref_pose[12];
+    int64_t mask_id = -1;             // pooled image used as mask (-1: none)
     std::vector<double> view_poses;   // N*12
     int W = 0, H = 0;
 
@@ -88,6 +90,10 @@ struct pmp_ctx {
     float* out_depth = nullptr;
     float* out_score = nullptr;
     int32_t* out_nghbr = nullptr;
+    uint32_t* views_q = nullptr;      // packed neighbours of the current keyframe, contiguous
+    size_t views_q_cap = 0;           // in words
+    std::vector<uint64_t> packed_key; // (id, generation) pairs + edge flag of views_q
+    uint64_t next_generation = 1;
     float* tmp0 = nullptr;
     float* tmp1 = nullptr;
     uint8_t* mask = nullptr;
@@ -116,6 +122,13 @@ struct pmp_ctx {
     double* prune_poses = nullptr;
     size_t prune_cap = 0;
 
+    // stage timing (HIP events on `stream`)
+    bool timing = false;
+    std::vector<hipEvent_t> events;   // pairs: start, end
+    std::vector<int> event_stage;     // stage of each pair
+    size_t events_used = 0;           // pairs recorded
+    int densify_calls = 0;
+
     // debug
     float* dbg = nullptr;
     size_t dbg_cap = 0;
@@ -124,6 +137,10 @@ struct pmp_ctx {
 namespace pmp {
 
 int set_error(pmp_ctx* ctx, int code, const std::string& msg);
+// Stage timing: returns the pair index to close with timing_end (or -1 when off).
+enum { kStageInit = 0, kStagePropagate = 1, kStageFinalize = 2 };
+int timing_begin(pmp_ctx* ctx, int stage);
+void timing_end(pmp_ctx* ctx, int pair);
 int check_hip(pmp_ctx* ctx, hipError_t e, const char* what);
 int ensure_workspace(pmp_ctx* ctx, int W, int H);
 int ensure_tables(pmp_ctx* ctx, int W, int H, float sigma_color);

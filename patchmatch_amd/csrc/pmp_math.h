@@ -35,8 +35,11 @@ PMP_DF float fma_(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 PMP_DF float abs_(float x) { return __builtin_fabsf(x); }
 PMP_DF float floor_(float x) { return __builtin_floorf(x); }
 
-// clamp with NaN -> lo (comparisons and selects only)
-PMP_DF float clampf(float x, float lo, float hi) { return (x >= lo) ? ((x <= hi) ? x : hi) : lo; }
+// clamp with NaN -> lo: v_max_f32 returns the non-NaN operand (IEEE mode), so
+// min(max(x, lo), hi) equals (x >= lo) ? ((x <= hi) ? x : hi) : lo for every x
+// (the two forms can differ only in the sign of a zero result when lo or hi
+// is zero, which no caller distinguishes)
+PMP_DF float clampf(float x, float lo, float hi) { return __builtin_fminf(__builtin_fmaxf(x, lo), hi); }
 
 constexpr float kPi = 3.14159265358979323846f;
 constexpr float kPiO2 = 1.57079632679489661923f;
@@ -69,9 +72,10 @@ PMP_DF float rsqrt_pos(float a) {
 
 // sqrt(x): x < 1e-30 (and NaN) -> 0, x > 1e30 -> 1e15
 PMP_DF float sqrt_(float x) {
-    if (!(x >= 1e-30f)) return 0.0f;
-    float a = (x <= 1e30f) ? x : 1e30f;
-    return a * rsqrt_pos(a);
+    const bool tiny = !(x >= 1e-30f);
+    const float a = tiny ? 1.0f : ((x <= 1e30f) ? x : 1e30f);
+    const float r = a * rsqrt_pos(a);
+    return tiny ? 0.0f : r;
 }
 
 PMP_DF float asin_core(float a) {
@@ -84,17 +88,14 @@ PMP_DF float asin_core(float a) {
     return fma_(a * z, p, a);
 }
 
-// asin(x), x in [-1, 1]
+// asin(x), x in [-1, 1]; one core polynomial, the square root only for |x| > 1/2
 PMP_DF float asin_(float x) {
-    float a = abs_(x);
-    float r;
-    if (a <= 0.5f) {
-        r = asin_core(a);
-    } else {
-        float z = (1.0f - a) * 0.5f;
-        float s = sqrt_(z);
-        r = fma_(-2.0f, asin_core(s), kPiO2);
-    }
+    const float a = abs_(x);
+    const bool big = a > 0.5f;
+    float in = a;
+    if (big) in = sqrt_((1.0f - a) * 0.5f);
+    const float c = asin_core(in);
+    const float r = big ? fma_(-2.0f, c, kPiO2) : c;
     return (x < 0.0f) ? -r : r;
 }
 
@@ -130,7 +131,8 @@ PMP_DF float atan2_(float y, float x) {
     float ay = abs_(y), ax = abs_(x);
     float mn = (ay < ax) ? ay : ax;
     float mx = (ay < ax) ? ax : ay;
-    float q = (mx >= 1e-30f) ? mn * rcp_pos(clampf(mx, 1e-30f, 1e30f)) : 0.0f;
+    float q = mn * rcp_pos(clampf(mx, 1e-30f, 1e30f));
+    q = (mx >= 1e-30f) ? q : 0.0f;
     q = (q <= 1.0f) ? q : 1.0f;
     float a = atan_core(q);
     if (ay > ax) a = kPiO2 - a;

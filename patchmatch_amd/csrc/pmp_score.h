@@ -98,33 +98,44 @@ PMP_DF float ncc(const RefStat& rs, float floor_, float Swy, float Swyy, float S
 }
 
 // Neighbour sample at (u, v); texel k at integer k (PatchMatchCuda.cu:146 looks
-// up (u+0.5, v+0.5) with texel centres at k+0.5).
-PMP_DF float sample(const uint32_t* __restrict__ q, int W, int H, int edge, float u, float v) {
-    float fx, fy;
-    uint32_t word;
-    if (edge == PMP_EDGE_ZERO) {
-        if (!(u >= 0.0f && u < (float)(W - 1) && v >= 0.0f && v < (float)(H - 1))) return 0.0f;
-        float x0f = floor_(u), y0f = floor_(v);
-        fx = u - x0f;
-        fy = v - y0f;
-        word = q[(size_t)(int)y0f * W + (int)x0f];
+// up (u+0.5, v+0.5) with texel centres at k+0.5).  The packed words of all
+// views are read through one buffer resource with 32-bit byte offsets
+// (view v starts at word v*W*H; offsets past the buffer read 0).
+//   kZero: CPU semantics (depthmap.cc:36-38), 0 outside [0,W-1) x [0,H-1);
+//   else:  u clamped to [0, umax] (umax = W with longitude wrap packed into
+//          the words, W-1 for clamp), v clamped to [0, H-1].
+template <bool kZero>
+PMP_DF float sample(const KfArgs& a, __amdgpu_buffer_rsrc_t rsrc, uint32_t view_off, float u, float v) {
+    const int W = a.W;
+    float uu, vv;
+    bool inside = true;
+    if constexpr (kZero) {
+        inside = (u >= 0.0f) && (u < (float)(W - 1)) && (v >= 0.0f) && (v < a.vmax);
+        uu = inside ? u : 0.0f;
+        vv = inside ? v : 0.0f;
     } else {
-        float uu = clampf(u, 0.0f, edge == PMP_EDGE_WRAP ? (float)W : (float)(W - 1));
-        float vv = clampf(v, 0.0f, (float)(H - 1));
-        float x0f = floor_(uu), y0f = floor_(vv);
-        fx = uu - x0f;
-        fy = vv - y0f;
-        int x0 = (int)x0f;
-        if (x0 >= W) x0 -= W;
-        word = q[(size_t)(int)y0f * W + x0];
+        uu = clampf(u, 0.0f, a.umax);
+        vv = clampf(v, 0.0f, a.vmax);
     }
-    float b0 = (float)(word & 0xFFu);
-    float b1 = (float)((word >> 8) & 0xFFu);
-    float b2 = (float)((word >> 16) & 0xFFu);
-    float b3 = (float)(word >> 24);
-    float top = fma_(fx, b1 - b0, b0);
-    float bot = fma_(fx, b3 - b2, b2);
-    return fma_(fy, bot - top, top);
+    const float x0f = floor_(uu), y0f = floor_(vv);
+    const float fx = uu - x0f, fy = vv - y0f;
+    int x0 = (int)x0f;
+    if constexpr (!kZero) x0 = (x0 >= W) ? x0 - W : x0;
+    const uint32_t idx = view_off + (uint32_t)((int)y0f * W + x0);
+    const uint32_t word = __builtin_amdgcn_raw_buffer_load_b32(rsrc, idx * 4u, 0, 0);
+    const float b0 = (float)(word & 0xFFu);
+    const float b1 = (float)((word >> 8) & 0xFFu);
+    const float b2 = (float)((word >> 16) & 0xFFu);
+    const float b3 = (float)(word >> 24);
+    const float top = fma_(fx, b1 - b0, b0);
+    const float bot = fma_(fx, b3 - b2, b2);
+    const float val = fma_(fy, bot - top, top);
+    if constexpr (kZero) return inside ? val : 0.0f;
+    return val;
+}
+
+PMP_DF __amdgpu_buffer_rsrc_t quad_rsrc(const KfArgs& a) {
+    return __builtin_amdgcn_make_buffer_rsrc((void*)a.quads, (short)0, (int)a.quads_bytes, 0x00020000);
 }
 
 // Equirectangular projection (PatchMatchCuda.cu:132-144).
@@ -174,14 +185,17 @@ PMP_DF float aggregate(const KfArgs& a, float (&sv)[NV]) {
     return s;
 }
 
-// Score of plane (px, py, pz) at pixel (i, j).
-template <int NV>
-PMP_DF float score_plane(const KfArgs& a, const float* sWc, const RefStat& rs, int i, int j, float px, float py,
-                         float pz) {
+// Moments of views [V0, V0+G) (G <= 3 live views keep the per-view
+// transforms in scalar registers).  Per view the taps are accumulated in tap
+// order, exactly as in a single pass over all views.
+template <int NV, int V0, int G, bool kZero>
+PMP_DF void view_group(const KfArgs& a, const float* sWc, const RefStat& rs, int i, int j, float px, float py,
+                       float pz, float (&sv)[NV]) {
     const int W = a.W, H = a.H;
-    float acc0[NV], acc1[NV], acc2[NV], shift[NV];
+    const __amdgpu_buffer_rsrc_t rsrc = quad_rsrc(a);
+    float acc0[G], acc1[G], acc2[G], shift[G];
 #pragma unroll
-    for (int v = 0; v < NV; ++v) { acc0[v] = 0.0f; acc1[v] = 0.0f; acc2[v] = 0.0f; shift[v] = 0.0f; }
+    for (int g = 0; g < G; ++g) { acc0[g] = 0.0f; acc1[g] = 0.0f; acc2[g] = 0.0f; shift[g] = 0.0f; }
     int t = 0;
     for (int k1 = -a.half; k1 <= a.half; k1 += a.stride) {
         const int ii = i + k1;
@@ -198,25 +212,43 @@ PMP_DF float score_plane(const KfArgs& a, const float* sWc, const RefStat& rs, i
             const float w = sWc[(int)abs_(xs)] * a.Ws[t];
             const float wxc = w * (xs - rs.mx);
 #pragma unroll
-            for (int v = 0; v < NV; ++v) {
+            for (int g = 0; g < G; ++g) {
+                const int v = V0 + g;
                 const float x = fma_(a.M[v][0], xc, fma_(a.M[v][1], yc, fma_(a.M[v][2], zc, a.c[v][0])));
                 const float y = fma_(a.M[v][3], xc, fma_(a.M[v][4], yc, fma_(a.M[v][5], zc, a.c[v][1])));
                 const float z = fma_(a.M[v][6], xc, fma_(a.M[v][7], yc, fma_(a.M[v][8], zc, a.c[v][2])));
                 float u, vv;
                 project(a, x, y, z, u, vv);
-                const float val = sample(a.quads[v], W, H, a.edge_mode, u, vv);
-                if (t == 0) shift[v] = val;
-                const float yd = val - shift[v];
+                const float val = sample<kZero>(a, rsrc, (uint32_t)(v * W * H), u, vv);
+                shift[g] = (t == 0) ? val : shift[g];
+                const float yd = val - shift[g];
                 const float tw = w * yd;
-                acc0[v] = acc0[v] + tw;
-                acc1[v] = fma_(tw, yd, acc1[v]);
-                acc2[v] = fma_(wxc, yd, acc2[v]);
+                acc0[g] = acc0[g] + tw;
+                acc1[g] = fma_(tw, yd, acc1[g]);
+                acc2[g] = fma_(wxc, yd, acc2[g]);
             }
         }
     }
-    float sv[NV];
 #pragma unroll
-    for (int v = 0; v < NV; ++v) sv[v] = ncc(rs, a.var_floor, acc0[v], acc1[v], acc2[v]);
+    for (int g = 0; g < G; ++g) sv[V0 + g] = ncc(rs, a.var_floor, acc0[g], acc1[g], acc2[g]);
+}
+
+template <int NV, int V0, bool kZero>
+PMP_DF void view_groups(const KfArgs& a, const float* sWc, const RefStat& rs, int i, int j, float px, float py,
+                        float pz, float (&sv)[NV]) {
+    if constexpr (V0 < NV) {
+        constexpr int G = (NV - V0) < 3 ? (NV - V0) : 3;
+        view_group<NV, V0, G, kZero>(a, sWc, rs, i, j, px, py, pz, sv);
+        view_groups<NV, V0 + G, kZero>(a, sWc, rs, i, j, px, py, pz, sv);
+    }
+}
+
+// Score of plane (px, py, pz) at pixel (i, j); kZero selects PMP_EDGE_ZERO sampling.
+template <int NV, bool kZero>
+PMP_DF float score_plane(const KfArgs& a, const float* sWc, const RefStat& rs, int i, int j, float px, float py,
+                         float pz) {
+    float sv[NV];
+    view_groups<NV, 0, kZero>(a, sWc, rs, i, j, px, py, pz, sv);
     return aggregate<NV>(a, sv);
 }
 

@@ -200,7 +200,7 @@ class Keyframe:
 
 
 def make_keyframe(seed: int, W: int, H: int, n_views: int, baseline=(0.1, 0.3),
-                  spheres: int = 0, tube: bool = False, mask_frac: float = 0.15) -> Keyframe:
+                  spheres: int = 0, tube: bool = False, mask_frac: float = 0.15, workers: int = 8) -> Keyframe:
     """Reference camera near the room centre plus n_views neighbours at random
     baselines/yaws (seeded)."""
     s_scene, s_pose, s_photo = _split_seeds(seed ^ 0x5EED, 3)
@@ -216,9 +216,22 @@ def make_keyframe(seed: int, W: int, H: int, n_views: int, baseline=(0.1, 0.3),
         d = np.array([np.cos(el) * np.sin(az), np.sin(el), np.cos(el) * np.cos(az)])
         poses.append(yaw_pose(rng.uniform(-np.pi, np.pi), c0 + L * d))
     prng = np.random.default_rng(s_photo)
-    img, gt = render(scene, ref_pose, W, H)
+    allposes = [ref_pose] + poses
+    if W * H * len(allposes) >= 4_000_000 and workers > 1:
+        # noise-free renders in worker processes (spawned: no inherited GPU
+        # state); photometric noise afterwards, in order, so results do not
+        # depend on the worker count
+        import concurrent.futures as cf
+        import multiprocessing as mp
+        with cf.ProcessPoolExecutor(max_workers=min(workers, len(allposes)),
+                                    mp_context=mp.get_context("spawn")) as ex:
+            rendered = list(ex.map(render, [scene] * len(allposes), allposes, [W] * len(allposes),
+                                   [H] * len(allposes)))
+    else:
+        rendered = [render(scene, p, W, H) for p in allposes]
+    img, gt = rendered[0]
     ref = photometric(img, prng)
-    views = [photometric(render(scene, p, W, H)[0], prng) for p in poses]
+    views = [photometric(r[0], prng) for r in rendered[1:]]
     mask = np.ones((H, W), np.uint8)
     mask[int(np.floor((1.0 - mask_frac) * H)):, :] = 0
     bgr = np.repeat(ref[:, :, None], 3, axis=2)
@@ -230,9 +243,9 @@ def config0(seed: int = 0) -> Keyframe:
     return make_keyframe(seed, 480, 240, 2, baseline=(0.1, 0.3))
 
 
-def config1(seed: int = 0, W: int = 1920, H: int = 960, n_views: int = 6) -> Keyframe:
+def config1(seed: int = 0, W: int = 1920, H: int = 960, n_views: int = 6, workers: int = 8) -> Keyframe:
     """1920x960, reference + 6 neighbours, room + 20 spheres + tube, baselines 0.05-0.5 m."""
-    return make_keyframe(seed, W, H, n_views, baseline=(0.05, 0.5), spheres=20, tube=True)
+    return make_keyframe(seed, W, H, n_views, baseline=(0.05, 0.5), spheres=20, tube=True, workers=workers)
 
 
 def trajectory(seed: int, n: int, spacing: float = 0.2):
