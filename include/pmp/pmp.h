@@ -221,6 +221,11 @@ int pmp_debug_eval_math(pmp_ctx* ctx, int32_t which, int64_t n, const float* a, 
 int pmp_debug_refine_normals(pmp_ctx* ctx, const pmp_params* params, int32_t width, int32_t i, int32_t j,
                              int32_t it, int32_t n, float* out);
 
+/* Copy `bytes` (multiple of 16) between two context-owned device buffers with
+ * a streaming kernel, `repeats` times on the context stream: a launch of known
+ * traffic for calibrating hardware byte counters. */
+int pmp_debug_copy(pmp_ctx* ctx, int64_t bytes, int32_t repeats);
+
 #ifdef __cplusplus
 }
 #endif

@@ -43,7 +43,7 @@ EXPORTS = [
     "pmp_densify", "pmp_set_maps", "pmp_get_maps", "pmp_eval_counts", "pmp_clean",
     "pmp_pruner_add_view", "pmp_pruner_prune", "pmp_pruner_pop_head", "pmp_pruner_reset",
     "pmp_pruner_size", "pmp_pruner_get_map", "pmp_debug_eval_math", "pmp_debug_refine_normals",
-    "pmp_set_timing", "pmp_get_timing", "pmp_set_mask",
+    "pmp_set_timing", "pmp_get_timing", "pmp_set_mask", "pmp_debug_copy",
 ]
 
 
@@ -98,6 +98,7 @@ def load():
         "pmp_set_timing": ([_vp, i32], c_int),
         "pmp_get_timing": ([_vp, P(Timing)], c_int),
         "pmp_set_mask": ([_vp, i64], c_int),
+        "pmp_debug_copy": ([_vp, i64, i32], c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -286,6 +287,9 @@ class Context:
         self._chk(self.L.pmp_debug_eval_math(self.h, self.MATH[name], a.size, _p(a, _f32p), _p(bb, _f32p),
                                              _p(out, _f32p)), "pmp_debug_eval_math")
         return out
+
+    def debug_copy(self, nbytes: int, repeats: int = 1):
+        self._chk(self.L.pmp_debug_copy(self.h, nbytes, repeats), "pmp_debug_copy")
 
     def refine_normals(self, p: Params, W, i, j, it, n=18):
         out = np.zeros(n, np.float32)

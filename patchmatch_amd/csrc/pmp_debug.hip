@@ -37,6 +37,11 @@ __global__ void k_refine_normals(KfArgs a, int i, int j, int it, int n, float* _
     for (int t = 0; t < n; ++t) out[t] = z[t];
 }
 
+__global__ __launch_bounds__(256) void k_copy(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n) {
+    for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x)
+        dst[k] = src[k];
+}
+
 static int ensure_dbg(pmp_ctx* ctx, size_t floats) {
     if (ctx->dbg_cap >= floats) return PMP_OK;
     if (ctx->dbg) (void)hipFree(ctx->dbg);
@@ -67,6 +72,20 @@ int pmp_debug_eval_math(pmp_ctx* ctx, int32_t which, int64_t n, const float* a, 
                        db, dout);
     PMP_HIP(ctx, hipGetLastError());
     PMP_HIP(ctx, hipMemcpyAsync(out, dout, n * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    PMP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return PMP_OK;
+}
+
+int pmp_debug_copy(pmp_ctx* ctx, int64_t bytes, int32_t repeats) {
+    if (!ctx || bytes <= 0 || bytes % 16 != 0 || bytes > (1ll << 33) || repeats < 1) return PMP_EINVAL;
+    int rc = ensure_dbg(ctx, (size_t)bytes / 2);   // two halves of bytes each
+    if (rc) return rc;
+    const size_t n = (size_t)bytes / 16;
+    const uint4* src = reinterpret_cast<const uint4*>(ctx->dbg);
+    uint4* dst = reinterpret_cast<uint4*>(ctx->dbg) + n;
+    for (int r = 0; r < repeats; ++r)
+        hipLaunchKernelGGL(k_copy, dim3(4096), dim3(256), 0, ctx->stream, src, dst, n);
+    PMP_HIP(ctx, hipGetLastError());
     PMP_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return PMP_OK;
 }

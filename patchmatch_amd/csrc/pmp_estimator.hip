@@ -24,9 +24,6 @@
 
 namespace pmp {
 
-constexpr int kBlockX = 64;   // one wave across a row of same-colour pixels
-constexpr int kBlockY = 4;
-
 __device__ __forceinline__ void load_wc(const KfArgs& a, float* sWc) {
     const int tid = threadIdx.y * blockDim.x + threadIdx.x;
     for (int k = tid; k < 256; k += blockDim.x * blockDim.y) sWc[k] = a.Wc[k];
@@ -53,15 +50,29 @@ __global__ void k_pack(const uint8_t* __restrict__ g, uint32_t* __restrict__ q, 
                            ((uint32_t)r1[x1] << 24);
 }
 
+// ---------------------------------------------------------------- tiles
+// Workgroups are dispatched to the 8 XCDs round-robin by linear block id.  The
+// grid is padded to a multiple of 8 and block b is given tile
+// (b mod 8) * (nb8/8) + b/8, so each XCD walks one contiguous band of tiles
+// and its L2 holds that band's neighbour rows.  Returns -1 for padding blocks.
+__device__ __forceinline__ int xcd_tile(int b, int nb8, int nb) {
+    const int t = (b & 7) * (nb8 >> 3) + (b >> 3);
+    return t < nb ? t : -1;
+}
+
 // ---------------------------------------------------------------- init
+// Tiles of 64 x 4 pixels, one thread per pixel.
 template <int NV, bool kZero>
 __global__ __launch_bounds__(256) void k_init(KfArgs a, float4* __restrict__ plane, float* __restrict__ depth,
                                               float* __restrict__ score, float4* __restrict__ iplane,
-                                              float* __restrict__ idepth, float* __restrict__ iscore) {
+                                              float* __restrict__ idepth, float* __restrict__ iscore, int nbx,
+                                              int nb) {
     __shared__ float sWc[256];
+    const int tile = xcd_tile(blockIdx.x, gridDim.x, nb);
+    if (tile < 0) return;
     load_wc(a, sWc);
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    const int i = blockIdx.y * blockDim.y + threadIdx.y;
+    const int j = (tile % nbx) * 64 + (threadIdx.x & 63);
+    const int i = (tile / nbx) * 4 + (threadIdx.x >> 6);
     if (i >= a.H || j >= a.W) return;
     float p0 = 0.0f, p1 = 0.0f, p2 = 0.0f, dep = 0.0f;
     if (in_init_grid(a, i, j)) random_plane(a, i, j, p0, p1, p2, dep);
@@ -79,11 +90,13 @@ __global__ __launch_bounds__(256) void k_init(KfArgs a, float4* __restrict__ pla
 // ---------------------------------------------------------------- score field
 template <int NV, bool kZero>
 __global__ __launch_bounds__(256) void k_score(KfArgs a, const float4* __restrict__ plane,
-                                               float* __restrict__ score) {
+                                               float* __restrict__ score, int nbx, int nb) {
     __shared__ float sWc[256];
+    const int tile = xcd_tile(blockIdx.x, gridDim.x, nb);
+    if (tile < 0) return;
     load_wc(a, sWc);
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    const int i = blockIdx.y * blockDim.y + threadIdx.y;
+    const int j = (tile % nbx) * 64 + (threadIdx.x & 63);
+    const int i = (tile / nbx) * 4 + (threadIdx.x >> 6);
     if (i >= a.H || j >= a.W || !in_score_grid(a, i, j)) return;
     const size_t k = (size_t)i * a.W + j;
     RefStat rs = ref_stats(a, sWc, i, j);
@@ -92,37 +105,71 @@ __global__ __launch_bounds__(256) void k_score(KfArgs a, const float4* __restric
 }
 
 // ---------------------------------------------------------------- red-black pass
-// Thread (jj, i) updates pixel j = 2 jj + ((i + colour) & 1): every pixel of
-// colour (i+j)&1 == colour once.  Candidates are at odd distance (1 or 3), so
-// they all have the other colour and are not written by this launch.
+// A tile is 4 rows x 64 same-colour pixels (128 columns); thread (w, l) of the
+// 256-thread block updates row i = 4 by + w, column j = 2 (64 bx + l) + ((i +
+// colour) & 1), i.e. every pixel of colour (i+j)&1 == colour once.  Candidates
+// are at odd distance (1 or 3), so they have the other colour and are not
+// written by this launch.
+//
+// Propagation candidates that survive the exact pre-rejection of a pixel are
+// scored by whichever threads of the block are free: per candidate index the
+// surviving (pixel, plane) jobs are packed into the first waves through LDS,
+// scored there, and the raw scores handed back to the owning threads, which
+// apply the penalty and the acceptance test in candidate order.  A pixel's
+// sequence of decisions is unchanged; only the lanes that compute its scores
+// differ, so results are identical to the one-thread-per-pixel loop.
 template <int NV, bool kZero>
 __global__ __launch_bounds__(256) void k_propagate(KfArgs a, float4* __restrict__ plane,
                                                    float* __restrict__ depth, float* __restrict__ score, int it,
-                                                   int colour, unsigned long long* __restrict__ counters) {
+                                                   int colour, unsigned long long* __restrict__ counters, int nbx,
+                                                   int nb) {
     __shared__ float sWc[256];
+    __shared__ float4 s_job[256];
+    __shared__ float4 s_rs[256];
+    __shared__ float s_ic[256];
+    __shared__ float s_res[256];
+    __shared__ int s_cnt[4];
+    const int tile = xcd_tile(blockIdx.x, gridDim.x, nb);
+    if (tile < 0) return;
     load_wc(a, sWc);
-    const int i = blockIdx.y * blockDim.y + threadIdx.y;
-    const int j = 2 * (int)(blockIdx.x * blockDim.x + threadIdx.x) + ((i + colour) & 1);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tbx = tile % nbx, tby = tile / nbx;
+    const int i = tby * 4 + wave;
+    const int j = 2 * (tbx * 64 + lane) + ((i + colour) & 1);
+    const bool active = i < a.H && j < a.W && in_prop_grid(a, i, j);
+    const int W = a.W;
+    const size_t k = active ? (size_t)i * W + j : 0;
     unsigned int n_eval = 0, n_cand = 0;
-    if (i < a.H && j < a.W && in_prop_grid(a, i, j)) {
-        const int W = a.W;
-        const size_t k = (size_t)i * W + j;
-        const RefStat rs = ref_stats(a, sWc, i, j);
-        float sc = score[k];
+
+    RefStat rs = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    float sc = 0.0f, h0 = 0.0f, h1 = 0.0f, h2 = 0.0f, bx = 0.0f, by = 0.0f, bz = 0.0f, d0 = 1.0f, inv_d0 = 1.0f;
+    if (active) {
+        rs = ref_stats(a, sWc, i, j);
+        sc = score[k];
         const float4 pk = plane[k];
-        float h0 = pk.x, h1 = pk.y, h2 = pk.z;
-        float bx, by, bz;
+        h0 = pk.x; h1 = pk.y; h2 = pk.z;
         bearing(a, i, j, bx, by, bz);
-        const float d0 = depth[k];
-        const float inv_d0 = rcp(d0);
-        // 1. propagation (PatchMatchCuda.cu:311-338)
-        // candidate order (0,+1) (+1,0) (0,-1) (-1,0) (0,+3) (+3,0) (0,-3) (-3,0)
+        d0 = depth[k];
+        inv_d0 = rcp(d0);
+    }
+    s_rs[tid] = make_float4(rs.mx, rs.inv, rs.mxc, rs.vx);
+    s_ic[tid] = rs.Ic;
+    // jobs fill waves starting at a block-dependent wave, spreading the busy
+    // waves of co-resident blocks over the SIMDs
+    const int rot = 64 * (blockIdx.x & 3);
+
+    // 1. propagation (PatchMatchCuda.cu:311-338)
+    // candidate order (0,+1) (+1,0) (0,-1) (-1,0) (0,+3) (+3,0) (0,-3) (-3,0)
 #pragma unroll 1
-        for (int c = 0; c < 8; ++c) {
+    for (int c = 0; c < 8; ++c) {
+        bool need = false;
+        float4 pn = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        float f = 0.0f;
+        if (active) {
             const int dist = (c < 4) ? 1 : 3, dir = c & 3;
             const int di = (dir == 1) ? dist : (dir == 3) ? -dist : 0;
             const int dj = (dir == 0) ? dist : (dir == 2) ? -dist : 0;
-            const float4 pn = plane[(size_t)(i + di) * W + (j + dj)];
+            pn = plane[(size_t)(i + di) * W + (j + dj)];
             ++n_cand;
             const float s = fma_(pn.z, bz, fma_(pn.y, by, pn.x * bx));
             const float dn = rcp_pos(clampf(abs_(s), 1e-6f, 1e30f));
@@ -132,18 +179,49 @@ __global__ __launch_bounds__(256) void k_propagate(KfArgs a, float4* __restrict_
             const float ex = h0 - pn.x, ey = h1 - pn.y, ez = h2 - pn.z;
             const float nrm = sqrt_(fma_(ez, ez, fma_(ey, ey, ex * ex)));
             // acosf(prod/norm) is NaN, i.e. the candidate is rejected, unless |prod| <= norm
-            if (!(nrm > 0.0f && abs_(prod) <= nrm)) continue;
-            const float ratio = clampf(prod * rcp_pos(clampf(nrm, 1e-30f, 1e30f)), -1.0f, 1.0f);
-            const float ac = acos_(ratio);
-            const float fp = exp_((ac * ac) * -a.plane_penalty);
-            const float f = (1.0f - fd) * (1.0f - fp);
-            // |score| <= 1: the penalised score cannot exceed f
-            if (!(f > sc)) continue;
-            float sh = score_plane<NV, kZero>(a, sWc, rs, i, j, pn.x, pn.y, pn.z);
+            if (nrm > 0.0f && abs_(prod) <= nrm) {
+                const float ratio = clampf(prod * rcp_pos(clampf(nrm, 1e-30f, 1e30f)), -1.0f, 1.0f);
+                const float ac = acos_(ratio);
+                const float fp = exp_((ac * ac) * -a.plane_penalty);
+                f = (1.0f - fd) * (1.0f - fp);
+                // |score| <= 1: the penalised score cannot exceed f
+                need = f > sc;
+            }
+        }
+        // pack the jobs of this candidate index
+        const unsigned long long bal = __ballot(need);
+        if (lane == 0) s_cnt[wave] = __popcll(bal);
+        __syncthreads();
+        int off = 0, total = 0;
+#pragma unroll
+        for (int w = 0; w < 4; ++w) {
+            const int cw = s_cnt[w];
+            off += (w < wave) ? cw : 0;
+            total += cw;
+        }
+        if (need) {
+            const int slot = off + __popcll(bal & ((1ull << lane) - 1ull));
+            s_job[(slot + rot) & 255] = make_float4(pn.x, pn.y, pn.z, __int_as_float(tid));
+        }
+        __syncthreads();
+        if (((tid - rot) & 255) < total) {
+            const float4 jb = s_job[tid];
+            const int jl = __float_as_int(jb.w);
+            const int il = tby * 4 + (jl >> 6);
+            const int jc = 2 * (tbx * 64 + (jl & 63)) + ((il + colour) & 1);
+            const float4 q = s_rs[jl];
+            RefStat r2;
+            r2.Ic = s_ic[jl]; r2.mx = q.x; r2.inv = q.y; r2.mxc = q.z; r2.vx = q.w;
+            s_res[jl] = score_plane<NV, kZero>(a, sWc, r2, il, jc, jb.x, jb.y, jb.z);
+        }
+        __syncthreads();
+        if (need) {
+            const float sh = s_res[tid] * f;
             ++n_eval;
-            sh = sh * f;
             if (sh > sc) { h0 = pn.x; h1 = pn.y; h2 = pn.z; sc = sh; }
         }
+    }
+    if (active) {
         // 2. refinement (PatchMatchCuda.cu:340-395)
         float dcur = depth_of_plane(bx, by, bz, h0, h1, h2);
         float c0 = h0, c1 = h1, c2 = h2;
@@ -380,32 +458,46 @@ static int fill_args(pmp_ctx* ctx, const pmp_params* p, KfArgs& a) {
         PMP_LAUNCH_NV(N, false, KERNEL, GRID, BLOCK, __VA_ARGS__)        \
     }
 
+// 1-D grid of 256-thread blocks over tiles of 64 x 4, padded to a multiple of 8
+struct TileGrid {
+    int nbx, nb;
+    dim3 grid() const { return dim3((unsigned)((nb + 7) / 8 * 8)); }
+};
+static TileGrid tile_grid(int cols, int rows) {
+    TileGrid g;
+    g.nbx = (cols + 63) / 64;
+    g.nb = g.nbx * ((rows + 3) / 4);
+    return g;
+}
+
 static int launch_init(pmp_ctx* ctx, const KfArgs& a) {
-    dim3 blk(kBlockX, kBlockY), grd((a.W + kBlockX - 1) / kBlockX, (a.H + kBlockY - 1) / kBlockY);
+    const TileGrid tg = tile_grid(a.W, a.H);
+    const dim3 blk(256), grd = tg.grid();
     const int tp = timing_begin(ctx, kStageInit);
     PMP_DISPATCH_NV(a.N, k_init, grd, blk, a, ctx->work.plane, ctx->work.depth, ctx->work.score, ctx->init.plane,
-                    ctx->init.depth, ctx->init.score);
+                    ctx->init.depth, ctx->init.score, tg.nbx, tg.nb);
     timing_end(ctx, tp);
     PMP_HIP(ctx, hipGetLastError());
     return PMP_OK;
 }
 
 static int launch_score(pmp_ctx* ctx, const KfArgs& a) {
-    dim3 blk(kBlockX, kBlockY), grd((a.W + kBlockX - 1) / kBlockX, (a.H + kBlockY - 1) / kBlockY);
-    PMP_DISPATCH_NV(a.N, k_score, grd, blk, a, ctx->work.plane, ctx->work.score);
+    const TileGrid tg = tile_grid(a.W, a.H);
+    const dim3 blk(256), grd = tg.grid();
+    PMP_DISPATCH_NV(a.N, k_score, grd, blk, a, ctx->work.plane, ctx->work.score, tg.nbx, tg.nb);
     PMP_HIP(ctx, hipGetLastError());
     return PMP_OK;
 }
 
 static int launch_passes(pmp_ctx* ctx, const KfArgs& a, int it_begin, int it_end) {
     PMP_HIP(ctx, hipMemsetAsync(ctx->counters, 0, 2 * sizeof(unsigned long long), ctx->stream));
-    const int half_w = (a.W + 1) / 2;
-    dim3 blk(kBlockX, kBlockY), grd((half_w + kBlockX - 1) / kBlockX, (a.H + kBlockY - 1) / kBlockY);
+    const TileGrid tg = tile_grid((a.W + 1) / 2, a.H);
+    const dim3 blk(256), grd = tg.grid();
     for (int it = it_begin; it < it_end; ++it)
         for (int colour = 0; colour < 2; ++colour) {
             const int tp = timing_begin(ctx, kStagePropagate);
             PMP_DISPATCH_NV(a.N, k_propagate, grd, blk, a, ctx->work.plane, ctx->work.depth, ctx->work.score, it,
-                            colour, ctx->counters);
+                            colour, ctx->counters, tg.nbx, tg.nb);
             timing_end(ctx, tp);
             PMP_HIP(ctx, hipGetLastError());
         }

@@ -1,0 +1,52 @@
+"""Workload driver for rocprofv3: densify the 1920x960 + 6-view keyframe
+`--steps` times, then run a streaming copy of known size (calibration of the
+hardware byte counters: a copy of B bytes moves B bytes in and B bytes out).
+
+    python -m patchmatch_amd.profile_run --steps 2 --copy-mib 256
+"""
+from __future__ import annotations
+
+import argparse
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--copy-mib", type=int, default=256)
+    ap.add_argument("--width", type=int, default=1920)
+    ap.add_argument("--height", type=int, default=960)
+    ap.add_argument("--views", type=int, default=6)
+    args = ap.parse_args(argv)
+
+    from patchmatch_amd import synth
+    from patchmatch_amd.params import config1_params
+    kf = synth.config1(0, W=args.width, H=args.height, n_views=args.views)
+    print(f"[profile_run] rendered {args.width}x{args.height}+{args.views}", flush=True)
+
+    from patchmatch_amd._lib import Context
+    p = config1_params()
+    with Context(0) as ctx:
+        ctx.load_keyframe(kf.ref, kf.views, kf.ref_pose, kf.view_poses)
+        ctx.image_put(1000, kf.mask)
+        ctx.set_mask(1000)
+        for s in range(args.steps):
+            t = time.perf_counter()
+            ctx.densify(p)
+            ctx.synchronize()
+            e, c = ctx.eval_counts()
+            print(f"[profile_run] step {s}: {1e3 * (time.perf_counter() - t):.1f} ms, {e} evaluations", flush=True)
+        if args.copy_mib > 0:
+            ctx.debug_copy(args.copy_mib << 20, 1)
+            print(f"[profile_run] copy of {args.copy_mib} MiB done", flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
