@@ -40,6 +40,7 @@
 #include <stdio.h>
 #include <math.h>
 #include <float.h>
+#include <stddef.h>
 #include "or_api.h"
 #include "or_math.h"
 
@@ -983,5 +984,47 @@ EXPORT int or_refine_normals(const or_keyframe* kf, const or_params* p, int i, i
     if (derive(kf, p, &d)) return -1;
     refine_normals(&d, i, j, it, n, z);
     release(&d);
+    return 0;
+}
+
+/* warped equirect coordinates (u, v) of reference pixel (ii, jj) through
+ * plane p into view v (the per-tap arithmetic of score_plane_f) */
+EXPORT int or_warp_point(const or_keyframe* kf, const or_params* p, int ii, int jj, int v, const float* plane,
+                         float* uv) {
+    derived_t d;
+    if (derive(kf, p, &d)) return -1;
+    if (v < 0 || v >= d.N || ii < 0 || ii >= d.H || jj < 0 || jj >= d.W) { release(&d); return -1; }
+    float bx, by, bz;
+    bearing_f(&d, ii, jj, &bx, &by, &bz);
+    float s = fmaf(plane[2], bz, fmaf(plane[1], by, plane[0] * bx));
+    float rsv = or_rcp_pos(or_clamp(fabsf(s), 1e-6f, 1e30f));
+    if (s < 0.0f) rsv = -rsv;
+    float xc = bx * rsv, yc = by * rsv, zc = bz * rsv;
+    const float* M = d.M[v]; const float* c = d.c[v];
+    float x = fmaf(M[0], xc, fmaf(M[1], yc, fmaf(M[2], zc, c[0])));
+    float y = fmaf(M[3], xc, fmaf(M[4], yc, fmaf(M[5], zc, c[1])));
+    float z = fmaf(M[6], xc, fmaf(M[7], yc, fmaf(M[8], zc, c[2])));
+    project_f(&d, x, y, z, &uv[0], &uv[1]);
+    release(&d);
+    return 0;
+}
+
+/* view aggregation of n per-view scores under p (agg_mode, topk, fold_agree) */
+EXPORT float or_aggregate(const or_params* p, int n, const float* sv) {
+    derived_t d;
+    d.N = n;
+    int k = p->topk < 1 ? 1 : p->topk;
+    if (k > n) k = n;
+    d.topk = k;
+    d.inv_k = (float)(1.0 / (double)k);
+    return aggregate_f(&d, p, sv);
+}
+
+/* layout probe: sizeof(or_params) and the offsets of its first and last fields */
+EXPORT int or_params_layout(int64_t* out) {
+    out[0] = (int64_t)sizeof(or_params);
+    out[1] = (int64_t)offsetof(or_params, seed);
+    out[2] = (int64_t)offsetof(or_params, min_depth);
+    out[3] = (int64_t)offsetof(or_params, median_rel_thr);
     return 0;
 }

@@ -57,6 +57,12 @@ def lib():
         L.or_refine_normals.argtypes = [P(OrKeyframe), P(Params), ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                         ctypes.c_int, _f32p]
         L.or_tables.argtypes = [ctypes.c_int, ctypes.c_int, _f32p, _f32p]
+        L.or_warp_point.argtypes = [P(OrKeyframe), P(Params), ctypes.c_int, ctypes.c_int, ctypes.c_int, _f32p, _f32p]
+        L.or_warp_point.restype = ctypes.c_int
+        L.or_aggregate.argtypes = [P(Params), ctypes.c_int, _f32p]
+        L.or_aggregate.restype = ctypes.c_float
+        L.or_params_layout.argtypes = [_i64p]
+        L.or_params_layout.restype = ctypes.c_int
         L.or_derive_poses.argtypes = [_f64p, ctypes.c_int, _f64p, _f32p, _f32p]
         for name in ("or_init", "or_score_field", "or_score_field_f64", "or_propagate", "or_propagate_rows",
                      "or_finalize", "or_densify", "or_clean", "or_prune", "or_eval_math", "or_refine_normals",
@@ -234,3 +240,29 @@ def refine_normals(kf: Keyframe, p: Params, i, j, it, n=18):
     z = np.zeros(n, np.float32)
     _chk(lib().or_refine_normals(ctypes.byref(kf.c), ctypes.byref(p), i, j, it, n, _p(z, _f32p)), "refine_normals")
     return z
+
+
+def warp_point(kf: Keyframe, p: Params, i, j, view, plane3):
+    pl = np.ascontiguousarray(plane3, dtype=np.float32)
+    uv = np.zeros(2, np.float32)
+    _chk(lib().or_warp_point(ctypes.byref(kf.c), ctypes.byref(p), i, j, view, _p(pl, _f32p), _p(uv, _f32p)),
+         "warp_point")
+    return float(uv[0]), float(uv[1])
+
+
+def aggregate(p: Params, scores):
+    sv = np.ascontiguousarray(scores, dtype=np.float32)
+    return float(lib().or_aggregate(ctypes.byref(p), sv.size, _p(sv, _f32p)))
+
+
+def tables(W, H):
+    rt = np.zeros((H, 2), np.float32)
+    ct = np.zeros((W, 2), np.float32)
+    lib().or_tables(W, H, _p(rt, _f32p), _p(ct, _f32p))
+    return rt, ct
+
+
+def params_layout():
+    out = np.zeros(4, np.int64)
+    lib().or_params_layout(_p(out, _i64p))
+    return [int(x) for x in out]

@@ -195,3 +195,15 @@ def test_prune_sequence_bitwise(gpu_ctx):
         gpu_ctx.pruner_add_view(maps[nxt], traj[nxt], bgrs[nxt])
         queue.append(nxt)
         nxt += 1
+
+
+def test_golden_fixture_on_gpu(gpu_ctx):
+    from tests_golden_loader import load_fixture
+    fx, p = load_fixture()
+    gpu_ctx.load_keyframe(fx["ref"], [fx["view0"], fx["view1"]], fx["ref_pose"], fx["view_poses"])
+    gpu_ctx.densify(p, fx["mask"])
+    gp, gd, gs, gn = gpu_ctx.get_maps(2)
+    assert _same_bits(gd, fx["depth"]).all() and _same_bits(gs, fx["score"]).all()
+    assert _same_bits(gp[..., :3], fx["plane"]).all() and np.array_equal(gn, fx["nghbr"])
+    wp, wd, ws = gpu_ctx.get_maps(0)
+    assert _same_bits(wd, fx["raw_depth"]).all() and _same_bits(wp, fx["raw_plane"]).all()
