@@ -55,6 +55,26 @@ def build_hip(verbose: bool = False) -> str:
     return LIB
 
 
+EXAMPLES = ["facade_window"]
+
+
+def build_examples(verbose: bool = False) -> list:
+    """C++ programs over the C-ABI and the header-only facade (linked to libpmp.so)."""
+    out = []
+    for name in EXAMPLES:
+        src = os.path.join(HERE, "examples", name + ".cpp")
+        exe = os.path.join(HERE, "examples", name)
+        cmd = ["g++", "-std=c++17", "-O2", "-Wall", f"-I{os.path.join(ROOT, 'include')}", src, f"-L{HERE}", "-lpmp",
+               "-Wl,-rpath,$ORIGIN/..", "-Wl,-rpath-link,/opt/rocm/lib", "-o", exe]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"example {name} failed to build:\n{r.stdout}\n{r.stderr}")
+        out.append(exe)
+        if verbose:
+            print(f"built {exe}")
+    return out
+
+
 def build_oracle(verbose: bool = False) -> str:
     r = subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], capture_output=True, text=True)
     if r.returncode != 0:
@@ -66,5 +86,6 @@ def build_oracle(verbose: bool = False) -> str:
 
 if __name__ == "__main__":
     build_hip(verbose=True)
+    build_examples(verbose=True)
     build_oracle(verbose=True)
     sys.exit(0)

@@ -1,0 +1,58 @@
+"""The C++ facade (include/pmp/densifier.hpp) over a sliding 3-view window,
+as the densify thread drives it, equals direct C-ABI calls bit for bit."""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from patchmatch_amd import synth
+from patchmatch_amd.params import config0_params
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "patchmatch_amd", "examples", "facade_window")
+
+
+def test_facade_sliding_window(gpu_ctx, tmp_path):
+    if not os.path.exists(EXE):
+        pytest.fail("facade_window not built (run __graft_entry__.build())")
+    W, H = 160, 80
+    traj = synth.trajectory(7, 4, 0.25)
+    scene = synth.make_scene(70)
+    rng = np.random.default_rng(3)
+    imgs = [synth.photometric(synth.render(scene, pose, W, H)[0], rng) for pose in traj]
+    mask = np.ones((H, W), np.uint8)
+    mask[int(0.85 * H):] = 0
+    p = config0_params(iterations=2, seed=77, min_depth=0.05, max_depth=12.0)
+    inp = tmp_path / "in.bin"
+    with open(inp, "wb") as f:
+        f.write(np.array([W, H, len(imgs)], np.int32).tobytes())
+        f.write(ctypes.string_at(ctypes.addressof(p), ctypes.sizeof(p)))
+        for pose, img in zip(traj, imgs):
+            f.write(np.asarray(pose, np.float64).tobytes())
+            f.write(img.tobytes())
+            f.write(mask.tobytes())
+            f.write(np.array([0.05, 12.0], np.float64).tobytes())
+    out = tmp_path / "out.bin"
+    r = subprocess.run([EXE, str(inp), str(out)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    data = np.fromfile(out, dtype=np.uint8)
+    P = W * H
+    per = 3 * 4 * P
+    assert data.size == 2 * per                        # windows [0,1,2] and [1,2,3]
+    windows = [(1, [0, 2]), (2, [1, 3])]               # centre = reference, swapped order of the rest
+    for w, (ref, nb) in enumerate(windows):
+        chunk = data[w * per:(w + 1) * per]
+        fd = chunk[:4 * P].view(np.float32).reshape(H, W)
+        fs = chunk[4 * P:8 * P].view(np.float32).reshape(H, W)
+        fn = chunk[8 * P:].view(np.int32).reshape(H, W)
+        gpu_ctx.load_keyframe(imgs[ref], [imgs[k] for k in nb], traj[ref], traj[nb])
+        gpu_ctx.densify(p, mask)
+        _, gd, gs, gn = gpu_ctx.get_maps(2)
+        assert np.array_equal(fd.view(np.uint32), gd.view(np.uint32))
+        assert np.array_equal(fs.view(np.uint32), gs.view(np.uint32))
+        assert np.array_equal(fn, gn)
+        assert (gd > 0).mean() > 0.2
