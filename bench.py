@@ -28,6 +28,10 @@ sys.path.insert(0, ROOT)
 
 FP32_PEAK_TFLOPS = 157.3       # MI355X published FP32 peak
 HBM_PEAK_GBS = 8000.0
+# L2-to-memory bytes per k_propagate launch on the 1920x960 + 6-view workload,
+# from rocprofv3 FETCH_SIZE and WRITE_SIZE scaled by a copy of known size
+# (profiles/k_propagate_c1_1920x960_6views.md); None for other workloads.
+TRAFFIC_BYTES_PER_LAUNCH_C1 = int((1_385_000 * 2.00 + 43_430 * 1.00) * 1024)
 
 
 def ops_per_eval(T: int, N: int) -> int:
@@ -164,7 +168,8 @@ def main(argv=None):
         "accuracy_vs_ground_truth": acc,
         "roofline": {"bound": "mfma", "kernel": "k_propagate", "achieved": round(achieved_tflops, 3),
                      "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved_tflops / FP32_PEAK_TFLOPS, 4),
-                     "traffic": None, "ops_per_eval": opc, "evals_per_launch": int(evals_per_launch),
+                     "traffic": TRAFFIC_BYTES_PER_LAUNCH_C1 if (args.width, args.height, N) == (1920, 960, 6) else None,
+                     "traffic_unit": "bytes per launch (L2 misses, rocprofv3)", "ops_per_eval": opc, "evals_per_launch": int(evals_per_launch),
                      "avg_launch_ms": round(avg_launch_ms, 4), "launches": launches},
     }
 

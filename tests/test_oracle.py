@@ -346,3 +346,29 @@ def test_golden_fixture_reproduced():
     out = or_py.densify(kf, p, fx["mask"])
     for k in ("depth", "plane", "score", "nghbr", "raw_plane", "raw_depth", "raw_score"):
         assert np.array_equal(np.asarray(out[k]).view(np.uint8), np.asarray(fx[k]).view(np.uint8)), k
+
+
+# ------------------------------------------------------------------ coefficient tables
+def test_polynomial_coefficients_match_fit_and_each_other():
+    """The oracle's and the device's tables are the same float32 literals, and
+    they are the output of patchmatch_amd/csrc/gen_coeffs.py (within 2 ulp: the
+    least-squares solve may differ in the last bit across BLAS builds)."""
+    import importlib.util
+    import re
+    root = os.path.dirname(HERE)
+    spec = importlib.util.spec_from_file_location("gen_coeffs", os.path.join(root, "patchmatch_amd", "csrc",
+                                                                             "gen_coeffs.py"))
+    gc = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(gc)
+    fit = gc.coefficients()
+    lit = re.compile(r"-?\d\.\d{9}e[+-]\d{2}f")
+    dev = [float(x[:-1]) for x in lit.findall(open(os.path.join(root, "patchmatch_amd", "csrc", "pmp_math.h")).read())]
+    orc = [float(x[:-1]) for x in lit.findall(open(os.path.join(root, "oracle", "or_math.h")).read())]
+    assert dev == orc and len(dev) >= 30
+    table = np.float32(dev)
+    for name, v in fit.items():
+        for x in v:
+            if abs(x) in (0.0, 1.0, 0.5):
+                continue
+            near = np.min(np.abs(table - x) / np.spacing(np.float32(abs(x))))
+            assert near <= 2, (name, x)
