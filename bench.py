@@ -11,7 +11,15 @@ mask resident in HBM.
 
 Prints one JSON line (rank 0).  Extra fields: latency percentiles, accuracy
 against the scene's ground truth, the dominant kernel's roofline position and
-the CPU oracle timed on a bounded sample on this host.
+the CPU oracle timed on a bounded sample on this host; then the two other
+forms of the metric (patchmatch_amd/workloads.py):
+  "batched": 64 keyframes of mixed resolution (960x480 / 1920x960 /
+             2880x1440, weights 0.3/0.5/0.2) and view counts U{2..10}, batched
+             (one work-queue launch per stage, hipGraph, autotuned) vs plain;
+  "served":  Poisson request streams at 5, 15 and 30 requests/s (resolutions
+             up to 5760x2880, views U{2..10}), served (continuous batching,
+             LRU image pool) vs the plain per-request path, p50/p99 latency
+             from arrival to maps on the host.
 """
 from __future__ import annotations
 
@@ -70,6 +78,9 @@ def main(argv=None):
     ap.add_argument("--views", type=int, default=6)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline sample length (0: skip)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: min(16, available cores)")
+    ap.add_argument("--batched", type=int, default=64, help="keyframes of the batched workload (0: skip)")
+    ap.add_argument("--served-requests", type=int, default=40, help="requests per served rate (0: skip)")
+    ap.add_argument("--rates", type=str, default="5,15,30", help="served arrival rates (requests/s)")
     args = ap.parse_args(argv)
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -166,7 +177,7 @@ def main(argv=None):
         "evals_per_step": int(evals / args.steps),
         "candidates_per_step": int(cands / args.steps),
         "accuracy_vs_ground_truth": acc,
-        "roofline": {"bound": "mfma", "kernel": "k_propagate", "achieved": round(achieved_tflops, 3),
+        "roofline": {"bound": "mfma", "kernel": "k_propagate_q", "achieved": round(achieved_tflops, 3),
                      "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved_tflops / FP32_PEAK_TFLOPS, 4),
                      "traffic": TRAFFIC_BYTES_PER_LAUNCH_C1 if (args.width, args.height, N) == (1920, 960, 6) else None,
                      "traffic_unit": "bytes per launch (L2 misses, rocprofv3)", "ops_per_eval": opc, "evals_per_launch": int(evals_per_launch),
@@ -188,6 +199,23 @@ def main(argv=None):
                        f"evaluations per keyframe (init + passes, as counted on the GPU)"),
             "evals_per_s": round(eps, 1)}
     ctx.close()
+
+    if args.batched > 0 or args.served_requests > 0:
+        from patchmatch_amd import workloads
+        if args.batched > 0:
+            t = time.perf_counter()
+            st = workloads.Stream(args.seed + 1, workloads.BATCH_CLASSES)
+            print(f"[bench] batched workload rendered in {time.perf_counter() - t:.1f}s", file=sys.stderr, flush=True)
+            result["batched"] = workloads.run_batched(st, p, args.batched, args.seed + 2)
+            print(f"[bench] batched: {result['batched']}", file=sys.stderr, flush=True)
+            del st
+        if args.served_requests > 0:
+            t = time.perf_counter()
+            st = workloads.Stream(args.seed + 3, workloads.SERVE_CLASSES)
+            print(f"[bench] served workload rendered in {time.perf_counter() - t:.1f}s", file=sys.stderr, flush=True)
+            rates = [float(r) for r in args.rates.split(",") if r]
+            result["served"] = workloads.run_served(st, p, rates, args.served_requests, args.seed + 4)
+            print(f"[bench] served: {result['served']}", file=sys.stderr, flush=True)
     print(json.dumps(result))
     return 0
 

@@ -40,7 +40,8 @@ enum {
     PMP_ENOMEM = -2,     /* device or host allocation failed */
     PMP_EDEVICE = -3,    /* HIP runtime error */
     PMP_ENOTFOUND = -4,  /* unknown image id */
-    PMP_ESTATE = -5      /* call out of order (e.g. no views set) */
+    PMP_ESTATE = -5,     /* call out of order (e.g. no views set) */
+    PMP_EBUSY = -6       /* not finished within the time limit */
 };
 
 enum { PMP_AGG_REF_FOLD = 0, PMP_AGG_MAX = 1, PMP_AGG_TOPK_MEAN = 2 };
@@ -182,6 +183,120 @@ typedef struct pmp_timing {
 } pmp_timing;
 int pmp_set_timing(pmp_ctx* ctx, int32_t enable);
 int pmp_get_timing(pmp_ctx* ctx, pmp_timing* out);
+
+/* ---- many keyframes per launch (batched form) --------------------------- */
+
+#define PMP_MAX_BATCH 128
+
+/* One keyframe of a batch: pooled reference and neighbour images with their
+ * world->camera poses (the arguments of pmp_set_views), an optional pooled
+ * mask, and the keyframe's depth range.  The reference derives the range per
+ * keyframe from its landmarks (depthmap_computation_module.cc:123-131);
+ * min_depth/max_depth > 0 replace the params' values for this keyframe. */
+typedef struct pmp_keyframe {
+    uint64_t ref_id;
+    pmp_pose ref_pose;
+    int32_t n_views;
+    int32_t reserved;
+    uint64_t view_ids[PMP_MAX_VIEWS];
+    pmp_pose view_poses[PMP_MAX_VIEWS];
+    int64_t mask_id;          /* pooled mask image (0 = ignored pixel), -1: none */
+    float min_depth, max_depth;
+} pmp_keyframe;
+
+/* Result-invariant launch configuration of the batched kernels: every value
+ * gives bit-identical maps, only the speed differs. */
+typedef struct pmp_batch_config {
+    int32_t tile_rows;        /* rows per propagation tile (2, 4 or 8); the block has 64*tile_rows threads */
+    int32_t blocks_per_cu;    /* persistent propagation blocks per CU (>= 1) */
+    int32_t use_graph;        /* 1: replay the whole batch (init, 2*iterations passes, finalize) as one hipGraph */
+    int32_t xcd_bands;        /* 1: each XCD drains its own contiguous band of tiles first */
+} pmp_batch_config;
+
+/* Densify n <= PMP_MAX_BATCH keyframes of any sizes and view counts with one
+ * set of params: one work-queue launch per stage walks the (keyframe, tile)
+ * items of every keyframe.  Per keyframe the maps are bit-identical to
+ * pmp_set_views + pmp_densify with the same params.  Asynchronous on the
+ * context stream; results stay on the device until pmp_batch_get_maps. */
+int pmp_batch_densify(pmp_ctx* ctx, const pmp_params* params, int32_t n, const pmp_keyframe* kfs);
+/* Final maps of keyframe k of the last batch (host pointers, nullable):
+ * plane 4 floats per pixel, depth, score, nghbr (as PMP_MAPS_FINAL). */
+int pmp_batch_get_maps(pmp_ctx* ctx, int32_t k, float* plane4, float* depth, float* score, int32_t* nghbr);
+int pmp_batch_set_config(pmp_ctx* ctx, const pmp_batch_config* cfg);
+int pmp_batch_get_config(pmp_ctx* ctx, pmp_batch_config* cfg);
+/* Times one red-black pass of the batch under every configuration in the
+ * search space (tile rows x blocks per CU), keeps the fastest as the context's
+ * configuration and writes it to *best (nullable).  The batch's results are
+ * not kept (run pmp_batch_densify afterwards). */
+int pmp_batch_autotune(pmp_ctx* ctx, const pmp_params* params, int32_t n, const pmp_keyframe* kfs,
+                       pmp_batch_config* best);
+
+/* ---- served form: a request stream with continuous batching ------------- */
+
+/* The reference densifies one keyframe at a time on the densify thread
+ * (depthmap_computation_module.cc:374-413, one synchronous ComputePatchMatch
+ * per keyframe).  A server owns a device context, an on-device LRU image pool
+ * admitted against an HBM budget, and a scheduler thread: every scheduler
+ * step launches one stage (init, one red-black pass, or finalize) of every
+ * keyframe in flight, each at its own stage, so requests that arrive while
+ * others run join at the next step.  Per request the maps are bit-identical
+ * to pmp_densify with the same params. */
+typedef struct pmp_server pmp_server;
+
+typedef struct pmp_server_opts {
+    int32_t device;
+    int32_t max_slots;          /* keyframes in flight, 1..PMP_MAX_BATCH (0: 16) */
+    int64_t hbm_budget;         /* device bytes for pooled images (5 B/pixel) and keyframe maps
+                                   (84 B/pixel); 0: 64 GiB */
+    int32_t edge_mode;          /* PMP_EDGE_* of every request (neighbour packing is shared) */
+    int32_t reserved;
+    pmp_batch_config config;    /* tile_rows, blocks_per_cu, xcd_bands of the step kernels */
+} pmp_server_opts;
+
+/* One densification request: the keyframe (pooled ids, poses, mask id, depth
+ * range) plus host pixels for every id the pool does not hold yet (read only
+ * during pmp_server_submit; nullable for pooled ids). */
+typedef struct pmp_request {
+    pmp_keyframe kf;
+    int32_t width, height;
+    const uint8_t* ref_pixels;
+    const uint8_t* view_pixels[PMP_MAX_VIEWS];
+    const uint8_t* mask_pixels;
+} pmp_request;
+
+typedef struct pmp_request_info {
+    double submit_ms, admit_ms, done_ms;   /* server clock: ms since pmp_server_create */
+    int32_t steps;                          /* scheduler steps that carried this request */
+    int32_t width, height, n_views;
+} pmp_request_info;
+
+typedef struct pmp_server_info {
+    int64_t submitted, completed, failed, steps;
+    int64_t pool_images, pool_bytes, maps_bytes, uploads, evictions;
+    int32_t in_flight, queued, max_in_flight, reserved;
+} pmp_server_info;
+
+int pmp_server_create(pmp_server** out, const pmp_server_opts* opts);
+/* Finishes every submitted request, then releases the device. */
+void pmp_server_destroy(pmp_server* srv);
+const char* pmp_server_last_error(pmp_server* srv);
+/* Queues a request with its own params (edge_mode must be the server's).
+ * Missing images are uploaded (and packed) before it returns; waits while
+ * the budget cannot hold them next to the images of requests in flight;
+ * PMP_ENOMEM if it never could. */
+int pmp_server_submit(pmp_server* srv, const pmp_request* req, const pmp_params* params, uint64_t* ticket);
+/* PMP_OK once the request is done (or failed), PMP_EBUSY after timeout_ms
+ * (< 0: no limit). */
+int pmp_server_wait(pmp_server* srv, uint64_t ticket, int32_t timeout_ms);
+/* Copies the final maps (as PMP_MAPS_FINAL; nullable pointers) and the
+ * request's timings, then forgets the ticket.  A failed request returns its
+ * error status. */
+int pmp_server_fetch(pmp_server* srv, uint64_t ticket, float* plane4, float* depth, float* score, int32_t* nghbr,
+                     pmp_request_info* info);
+int pmp_server_stats(pmp_server* srv, pmp_server_info* out);
+/* paused != 0: admit no queued request until resumed (requests in flight
+ * continue); submissions are still accepted. */
+int pmp_server_pause(pmp_server* srv, int32_t paused);
 
 /* ---- geometric consistency (DepthmapCleaner, depthmap.cc:1320-1377) ----- */
 

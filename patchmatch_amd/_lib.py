@@ -16,7 +16,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libpmp.so")
 
 PMP_MAPS_WORK, PMP_MAPS_INIT, PMP_MAPS_FINAL = 0, 1, 2
-STATUS = {0: "PMP_OK", -1: "PMP_EINVAL", -2: "PMP_ENOMEM", -3: "PMP_EDEVICE", -4: "PMP_ENOTFOUND", -5: "PMP_ESTATE"}
+STATUS = {0: "PMP_OK", -1: "PMP_EINVAL", -2: "PMP_ENOMEM", -3: "PMP_EDEVICE", -4: "PMP_ENOTFOUND", -5: "PMP_ESTATE",
+          -6: "PMP_EBUSY"}
 
 _f32p = ctypes.POINTER(ctypes.c_float)
 _f64p = ctypes.POINTER(ctypes.c_double)
@@ -44,12 +45,70 @@ EXPORTS = [
     "pmp_pruner_add_view", "pmp_pruner_prune", "pmp_pruner_pop_head", "pmp_pruner_reset",
     "pmp_pruner_size", "pmp_pruner_get_map", "pmp_debug_eval_math", "pmp_debug_refine_normals",
     "pmp_set_timing", "pmp_get_timing", "pmp_set_mask", "pmp_debug_copy",
+    "pmp_batch_densify", "pmp_batch_get_maps", "pmp_batch_set_config", "pmp_batch_get_config",
+    "pmp_batch_autotune", "pmp_server_create", "pmp_server_destroy", "pmp_server_last_error",
+    "pmp_server_submit", "pmp_server_wait", "pmp_server_fetch", "pmp_server_stats", "pmp_server_pause",
 ]
+
+PMP_MAX_VIEWS = 10
+PMP_MAX_BATCH = 128
+
+
+class Keyframe(ctypes.Structure):
+    """pmp_keyframe: one keyframe of a batch (pooled image ids + poses)."""
+    _fields_ = [("ref_id", ctypes.c_uint64), ("ref_pose", Pose), ("n_views", ctypes.c_int32),
+                ("reserved", ctypes.c_int32), ("view_ids", ctypes.c_uint64 * PMP_MAX_VIEWS),
+                ("view_poses", Pose * PMP_MAX_VIEWS), ("mask_id", ctypes.c_int64),
+                ("min_depth", ctypes.c_float), ("max_depth", ctypes.c_float)]
+
+    @classmethod
+    def make(cls, ref_id, ref_pose, view_ids, view_poses, mask_id=-1, min_depth=0.0, max_depth=0.0):
+        k = cls()
+        k.ref_id = ref_id
+        k.ref_pose = Pose.from12(ref_pose)
+        k.n_views = len(view_ids)
+        for v, (vid, vp) in enumerate(zip(view_ids, view_poses)):
+            k.view_ids[v] = vid
+            k.view_poses[v] = Pose.from12(vp)
+        k.mask_id = mask_id
+        k.min_depth = min_depth
+        k.max_depth = max_depth
+        return k
+
+
+class BatchConfig(ctypes.Structure):
+    _fields_ = [("tile_rows", ctypes.c_int32), ("blocks_per_cu", ctypes.c_int32), ("use_graph", ctypes.c_int32),
+                ("xcd_bands", ctypes.c_int32)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
 
 
 class Timing(ctypes.Structure):
     _fields_ = [("init_ms", ctypes.c_double), ("propagate_ms", ctypes.c_double), ("finalize_ms", ctypes.c_double),
                 ("propagate_launches", ctypes.c_int32), ("densify_calls", ctypes.c_int32)]
+
+class ServerOpts(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int32), ("max_slots", ctypes.c_int32), ("hbm_budget", ctypes.c_int64),
+                ("edge_mode", ctypes.c_int32), ("reserved", ctypes.c_int32), ("config", BatchConfig)]
+
+
+class Request(ctypes.Structure):
+    _fields_ = [("kf", Keyframe), ("width", ctypes.c_int32), ("height", ctypes.c_int32),
+                ("ref_pixels", _u8p), ("view_pixels", _u8p * PMP_MAX_VIEWS), ("mask_pixels", _u8p)]
+
+
+class RequestInfo(ctypes.Structure):
+    _fields_ = [("submit_ms", ctypes.c_double), ("admit_ms", ctypes.c_double), ("done_ms", ctypes.c_double),
+                ("steps", ctypes.c_int32), ("width", ctypes.c_int32), ("height", ctypes.c_int32),
+                ("n_views", ctypes.c_int32)]
+
+
+class ServerInfo(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int64) for n in ("submitted", "completed", "failed", "steps", "pool_images",
+                                              "pool_bytes", "maps_bytes", "uploads", "evictions")] + \
+               [(n, ctypes.c_int32) for n in ("in_flight", "queued", "max_in_flight", "reserved")]
+
 
 _lib = None
 
@@ -99,6 +158,19 @@ def load():
         "pmp_get_timing": ([_vp, P(Timing)], c_int),
         "pmp_set_mask": ([_vp, i64], c_int),
         "pmp_debug_copy": ([_vp, i64, i32], c_int),
+        "pmp_batch_densify": ([_vp, P(Params), i32, P(Keyframe)], c_int),
+        "pmp_batch_get_maps": ([_vp, i32, _f32p, _f32p, _f32p, _i32p], c_int),
+        "pmp_batch_set_config": ([_vp, P(BatchConfig)], c_int),
+        "pmp_batch_get_config": ([_vp, P(BatchConfig)], c_int),
+        "pmp_batch_autotune": ([_vp, P(Params), i32, P(Keyframe), P(BatchConfig)], c_int),
+        "pmp_server_create": ([P(_vp), P(ServerOpts)], c_int),
+        "pmp_server_destroy": ([_vp], None),
+        "pmp_server_last_error": ([_vp], ctypes.c_char_p),
+        "pmp_server_submit": ([_vp, P(Request), P(Params), _u64p], c_int),
+        "pmp_server_wait": ([_vp, u64, i32], c_int),
+        "pmp_server_fetch": ([_vp, u64, _f32p, _f32p, _f32p, _i32p, P(RequestInfo)], c_int),
+        "pmp_server_stats": ([_vp, P(ServerInfo)], c_int),
+        "pmp_server_pause": ([_vp, i32], c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -231,6 +303,37 @@ class Context:
         self._chk(self.L.pmp_eval_counts(self.h, ctypes.byref(e), ctypes.byref(c)), "pmp_eval_counts")
         return e.value, c.value
 
+    # -- many keyframes per launch
+    def batch_densify(self, p: Params, kfs):
+        arr = (Keyframe * len(kfs))(*kfs)
+        self._chk(self.L.pmp_batch_densify(self.h, ctypes.byref(p), len(kfs), arr), "pmp_batch_densify")
+
+    def batch_get_maps(self, k: int, shape):
+        H, W = shape
+        plane = np.zeros((H, W, 4), np.float32)
+        depth = np.zeros((H, W), np.float32)
+        score = np.zeros((H, W), np.float32)
+        nghbr = np.zeros((H, W), np.int32)
+        self._chk(self.L.pmp_batch_get_maps(self.h, k, _p(plane, _f32p), _p(depth, _f32p), _p(score, _f32p),
+                                            _p(nghbr, _i32p)), "pmp_batch_get_maps")
+        return plane, depth, score, nghbr
+
+    def batch_set_config(self, tile_rows=4, blocks_per_cu=0, use_graph=1, xcd_bands=1):
+        c = BatchConfig(tile_rows, blocks_per_cu, use_graph, xcd_bands)
+        self._chk(self.L.pmp_batch_set_config(self.h, ctypes.byref(c)), "pmp_batch_set_config")
+
+    def batch_get_config(self) -> dict:
+        c = BatchConfig()
+        self._chk(self.L.pmp_batch_get_config(self.h, ctypes.byref(c)), "pmp_batch_get_config")
+        return c.as_dict()
+
+    def batch_autotune(self, p: Params, kfs) -> dict:
+        arr = (Keyframe * len(kfs))(*kfs)
+        c = BatchConfig()
+        self._chk(self.L.pmp_batch_autotune(self.h, ctypes.byref(p), len(kfs), arr, ctypes.byref(c)),
+                  "pmp_batch_autotune")
+        return c.as_dict()
+
     # -- cleaner / pruner
     def clean(self, depths, poses, thr, min_consistent):
         depths = [np.ascontiguousarray(d, dtype=np.float32) for d in depths]
@@ -296,3 +399,93 @@ class Context:
         self._chk(self.L.pmp_debug_refine_normals(self.h, ctypes.byref(p), W, i, j, it, n, _p(out, _f32p)),
                   "pmp_debug_refine_normals")
         return out
+
+
+class Server:
+    """pmp_server: densification requests served with continuous batching."""
+
+    def __init__(self, device=0, max_slots=16, hbm_budget=0, edge_mode=0, tile_rows=8, blocks_per_cu=0,
+                 xcd_bands=1):
+        self.L = load()
+        o = ServerOpts(device, max_slots, hbm_budget, edge_mode, 0, BatchConfig(tile_rows, blocks_per_cu, 1, xcd_bands))
+        h = _vp()
+        rc = self.L.pmp_server_create(ctypes.byref(h), ctypes.byref(o))
+        if rc != 0:
+            raise PmpError(f"pmp_server_create failed: {STATUS.get(rc, rc)} (is a GPU visible?)")
+        self.h = h
+        self.shapes = {}
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.pmp_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _chk(self, rc, what):
+        if rc != 0:
+            msg = self.L.pmp_server_last_error(self.h).decode(errors="replace")
+            raise PmpError(f"{what}: {STATUS.get(rc, rc)}: {msg}")
+
+    def submit(self, p: Params, ref_id, ref_pose, view_ids, view_poses, shape, ref=None, views=None, mask_id=-1,
+               mask=None, min_depth=0.0, max_depth=0.0) -> int:
+        """Queue one keyframe; arrays are needed only for ids the pool does not hold."""
+        H, W = shape
+        r = Request()
+        r.kf = Keyframe.make(ref_id, ref_pose, view_ids, view_poses, mask_id, min_depth, max_depth)
+        r.width, r.height = W, H
+        keep = []
+
+        def px(a):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a, dtype=np.uint8)
+            assert a.shape == (H, W)
+            keep.append(a)
+            return a.ctypes.data_as(_u8p)
+        r.ref_pixels = px(ref)
+        for v in range(len(view_ids)):
+            r.view_pixels[v] = px(None if views is None else views[v])
+        r.mask_pixels = px(mask)
+        t = ctypes.c_uint64()
+        self._chk(self.L.pmp_server_submit(self.h, ctypes.byref(r), ctypes.byref(p), ctypes.byref(t)),
+                  "pmp_server_submit")
+        self.shapes[t.value] = (H, W)
+        return t.value
+
+    def wait(self, ticket: int, timeout_ms: int = -1) -> bool:
+        rc = self.L.pmp_server_wait(self.h, ticket, timeout_ms)
+        if rc == -6:
+            return False
+        self._chk(rc, "pmp_server_wait")
+        return True
+
+    def fetch(self, ticket: int):
+        """(plane, depth, score, nghbr, info) of a finished request; forgets the ticket."""
+        H, W = self.shapes.pop(ticket)
+        plane = np.zeros((H, W, 4), np.float32)
+        depth = np.zeros((H, W), np.float32)
+        score = np.zeros((H, W), np.float32)
+        nghbr = np.zeros((H, W), np.int32)
+        info = RequestInfo()
+        self._chk(self.L.pmp_server_fetch(self.h, ticket, _p(plane, _f32p), _p(depth, _f32p), _p(score, _f32p),
+                                          _p(nghbr, _i32p), ctypes.byref(info)), "pmp_server_fetch")
+        return plane, depth, score, nghbr, {f: getattr(info, f) for f, _ in RequestInfo._fields_}
+
+    def pause(self, paused: bool = True):
+        self._chk(self.L.pmp_server_pause(self.h, 1 if paused else 0), "pmp_server_pause")
+
+    def stats(self) -> dict:
+        s = ServerInfo()
+        self._chk(self.L.pmp_server_stats(self.h, ctypes.byref(s)), "pmp_server_stats")
+        return {f: getattr(s, f) for f, _ in ServerInfo._fields_}

@@ -61,62 +61,130 @@ void derive_poses(const double* ref_pose, int N, const double* view_poses, float
     }
 }
 
+void free_slot(SlotBuf& b) {
+    void* p[] = {b.plane, b.iplane, b.oplane, b.depth, b.score, b.idepth, b.iscore, b.odepth, b.oscore,
+                 b.tmp0, b.tmp1, b.onghbr};
+    for (void* x : p)
+        if (x) (void)hipFree(x);
+    b = SlotBuf();
+}
+
 template <typename T>
-static int grow(pmp_ctx* ctx, T*& p, size_t n) {
-    if (p) (void)hipFree(p);
-    p = nullptr;
+static int alloc_n(pmp_ctx* ctx, T*& p, size_t n) {
     PMP_HIP(ctx, hipMalloc(&p, n * sizeof(T)));
+    return PMP_OK;
+}
+
+// Sizes b for a W x H keyframe; reallocates (after the caller has made sure no
+// queued work uses the old buffers) only when it grows.
+int ensure_slot(pmp_ctx* ctx, SlotBuf& b, int W, int H) {
+    const size_t P = (size_t)W * H;
+    if (P > b.cap) {
+        free_slot(b);
+        int rc = 0;
+        rc |= alloc_n(ctx, b.plane, P);
+        rc |= alloc_n(ctx, b.iplane, P);
+        rc |= alloc_n(ctx, b.oplane, P);
+        rc |= alloc_n(ctx, b.depth, P);
+        rc |= alloc_n(ctx, b.score, P);
+        rc |= alloc_n(ctx, b.idepth, P);
+        rc |= alloc_n(ctx, b.iscore, P);
+        rc |= alloc_n(ctx, b.odepth, P);
+        rc |= alloc_n(ctx, b.oscore, P);
+        rc |= alloc_n(ctx, b.tmp0, P);
+        rc |= alloc_n(ctx, b.tmp1, P);
+        rc |= alloc_n(ctx, b.onghbr, P);
+        if (rc) {
+            free_slot(b);
+            return rc;
+        }
+        b.cap = P;
+    }
+    b.W = W;
+    b.H = H;
     return PMP_OK;
 }
 
 int ensure_workspace(pmp_ctx* ctx, int W, int H) {
     const size_t P = (size_t)W * H;
-    if (P <= ctx->ws_pixels) return PMP_OK;
-    PMP_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    int rc = 0;
-    rc |= grow(ctx, ctx->work.plane, P);
-    rc |= grow(ctx, ctx->work.depth, P);
-    rc |= grow(ctx, ctx->work.score, P);
-    rc |= grow(ctx, ctx->init.plane, P);
-    rc |= grow(ctx, ctx->init.depth, P);
-    rc |= grow(ctx, ctx->init.score, P);
-    rc |= grow(ctx, ctx->out_plane, P);
-    rc |= grow(ctx, ctx->out_depth, P);
-    rc |= grow(ctx, ctx->out_score, P);
-    rc |= grow(ctx, ctx->out_nghbr, P);
-    rc |= grow(ctx, ctx->tmp0, P);
-    rc |= grow(ctx, ctx->tmp1, P);
-    rc |= grow(ctx, ctx->mask, P);
-    if (rc) return rc;
-    if (!ctx->counters) PMP_HIP(ctx, hipMalloc(&ctx->counters, 4 * sizeof(unsigned long long)));
-    PMP_HIP(ctx, hipMemsetAsync(ctx->counters, 0, 4 * sizeof(unsigned long long), ctx->stream));
-    ctx->ws_pixels = P;
+    if (P > ctx->ws.cap) {
+        PMP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        if (ctx->mask) (void)hipFree(ctx->mask);
+        ctx->mask = nullptr;
+        int rc = ensure_slot(ctx, ctx->ws, W, H);
+        if (rc) return rc;
+        if ((rc = alloc_n(ctx, ctx->mask, P))) return rc;
+    }
+    ctx->ws.W = W;
+    ctx->ws.H = H;
+    if (!ctx->counters) {
+        PMP_HIP(ctx, hipMalloc(&ctx->counters, 4 * sizeof(unsigned long long)));
+        PMP_HIP(ctx, hipMemsetAsync(ctx->counters, 0, 4 * sizeof(unsigned long long), ctx->stream));
+    }
     return PMP_OK;
 }
 
-int ensure_tables(pmp_ctx* ctx, int W, int H, float sigma_color) {
-    if (ctx->tab_W != W || ctx->tab_H != H || !ctx->rowtab) {
+int get_tables(pmp_ctx* ctx, int W, int H, float sigma_color, Tables& t) {
+    auto ir = ctx->rowtabs.find(H);
+    auto ic = ctx->coltabs.find(W);
+    if (ir == ctx->rowtabs.end() || ic == ctx->coltabs.end()) {
         std::vector<float2> rt, ct;
         host_tables(W, H, rt, ct);
-        if (ctx->rowtab) (void)hipFree(ctx->rowtab);
-        if (ctx->coltab) (void)hipFree(ctx->coltab);
-        ctx->rowtab = nullptr;
-        ctx->coltab = nullptr;
-        PMP_HIP(ctx, hipMalloc(&ctx->rowtab, H * sizeof(float2)));
-        PMP_HIP(ctx, hipMalloc(&ctx->coltab, W * sizeof(float2)));
-        PMP_HIP(ctx, hipMemcpy(ctx->rowtab, rt.data(), H * sizeof(float2), hipMemcpyHostToDevice));
-        PMP_HIP(ctx, hipMemcpy(ctx->coltab, ct.data(), W * sizeof(float2), hipMemcpyHostToDevice));
-        ctx->tab_W = W;
-        ctx->tab_H = H;
+        if (ir == ctx->rowtabs.end()) {
+            float2* d = nullptr;
+            PMP_HIP(ctx, hipMalloc(&d, H * sizeof(float2)));
+            ctx->rowtabs[H] = d;
+            PMP_HIP(ctx, hipMemcpy(d, rt.data(), H * sizeof(float2), hipMemcpyHostToDevice));
+            ir = ctx->rowtabs.find(H);
+        }
+        if (ic == ctx->coltabs.end()) {
+            float2* d = nullptr;
+            PMP_HIP(ctx, hipMalloc(&d, W * sizeof(float2)));
+            ctx->coltabs[W] = d;
+            PMP_HIP(ctx, hipMemcpy(d, ct.data(), W * sizeof(float2), hipMemcpyHostToDevice));
+            ic = ctx->coltabs.find(W);
+        }
     }
-    if (ctx->wc_sigma != sigma_color || !ctx->wc) {
+    uint32_t key;
+    memcpy(&key, &sigma_color, 4);
+    auto iw = ctx->wcs.find(key);
+    if (iw == ctx->wcs.end()) {
         float h[256];
         const double cf = 1.0 / (2.0 * (double)sigma_color * (double)sigma_color);
         for (int a = 0; a < 256; ++a) h[a] = (float)exp(-(double)(a * a) * cf);
-        if (!ctx->wc) PMP_HIP(ctx, hipMalloc(&ctx->wc, 256 * sizeof(float)));
-        PMP_HIP(ctx, hipMemcpy(ctx->wc, h, sizeof(h), hipMemcpyHostToDevice));
-        ctx->wc_sigma = sigma_color;
+        float* d = nullptr;
+        PMP_HIP(ctx, hipMalloc(&d, sizeof(h)));
+        ctx->wcs[key] = d;
+        PMP_HIP(ctx, hipMemcpy(d, h, sizeof(h), hipMemcpyHostToDevice));
+        iw = ctx->wcs.find(key);
     }
+    t.row = ir->second;
+    t.col = ic->second;
+    t.wc = iw->second;
+    return PMP_OK;
+}
+
+// (x,y) (x+1,y) (x,y+1) (x+1,y+1) in bytes 0..3; x+1 of the last column wraps
+// to 0 (longitude wrap) or repeats the last column; y+1 of the last row repeats it.
+__global__ void k_pack(const uint8_t* __restrict__ g, uint32_t* __restrict__ q, int W, int H, int wrap) {
+    int x = blockIdx.x * blockDim.x + threadIdx.x;
+    int y = blockIdx.y;
+    if (x >= W || y >= H) return;
+    int x1 = x + 1 < W ? x + 1 : (wrap ? 0 : W - 1);
+    int y1 = y + 1 < H ? y + 1 : H - 1;
+    const uint8_t* r0 = g + (size_t)y * W;
+    const uint8_t* r1 = g + (size_t)y1 * W;
+    q[(size_t)y * W + x] = (uint32_t)r0[x] | ((uint32_t)r0[x1] << 8) | ((uint32_t)r1[x] << 16) |
+                           ((uint32_t)r1[x1] << 24);
+}
+
+int ensure_quads(pmp_ctx* ctx, Image& im, int wrap, hipStream_t stream) {
+    if (im.quads && im.quads_wrap == wrap) return PMP_OK;
+    if (!im.quads) PMP_HIP(ctx, hipMalloc(&im.quads, im.bytes * sizeof(uint32_t)));
+    dim3 blk(256), grd((im.W + 255) / 256, im.H);
+    hipLaunchKernelGGL(k_pack, grd, blk, 0, stream, im.gray, im.quads, im.W, im.H, wrap);
+    PMP_HIP(ctx, hipGetLastError());
+    im.quads_wrap = wrap;
     return PMP_OK;
 }
 
@@ -142,6 +210,7 @@ void timing_end(pmp_ctx* ctx, int pair) {
 
 static void free_image(Image& im) {
     if (im.gray) (void)hipFree(im.gray);
+    if (im.quads) (void)hipFree(im.quads);
     im = Image();
 }
 
@@ -172,15 +241,17 @@ void pmp_destroy(pmp_ctx* ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
+    queue_free(ctx);
+    free_slot(ctx->ws);
     for (auto& kv : ctx->images) free_image(kv.second);
-    void* bufs[] = {ctx->work.plane, ctx->work.depth, ctx->work.score, ctx->init.plane, ctx->init.depth,
-                    ctx->init.score, ctx->out_plane, ctx->out_depth, ctx->out_score, ctx->out_nghbr,
-                    ctx->tmp0, ctx->tmp1, ctx->mask, ctx->rowtab, ctx->coltab, ctx->wc, ctx->counters,
+    void* bufs[] = {ctx->mask, ctx->counters,
                     ctx->clean_maps, ctx->clean_poses, ctx->prune_z, ctx->prune_alive, ctx->prune_out,
-                    ctx->prune_pts, ctx->prune_cols, ctx->prune_kps, ctx->prune_blk, ctx->prune_poses, ctx->dbg,
-                    ctx->views_q};
+                    ctx->prune_pts, ctx->prune_cols, ctx->prune_kps, ctx->prune_blk, ctx->prune_poses, ctx->dbg};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
+    for (auto& kv : ctx->rowtabs) (void)hipFree(kv.second);
+    for (auto& kv : ctx->coltabs) (void)hipFree(kv.second);
+    for (auto& kv : ctx->wcs) (void)hipFree(kv.second);
     for (auto& v : ctx->pruner) {
         if (v.depth) (void)hipFree(v.depth);
         if (v.bgr) (void)hipFree(v.bgr);
@@ -271,7 +342,6 @@ int pmp_image_put(pmp_ctx* ctx, uint64_t id, const uint8_t* pixels, int32_t widt
     Image im;
     im.W = width;
     im.H = height;
-    im.generation = ctx->next_generation++;
     const size_t bytes = (size_t)width * height;
     PMP_HIP(ctx, hipMalloc(&im.gray, bytes));
     im.bytes = bytes;
@@ -304,7 +374,7 @@ int pmp_image_drop(pmp_ctx* ctx, uint64_t id) {
 int pmp_image_pool_stats(const pmp_ctx* ctx, int64_t* count, int64_t* bytes) {
     if (!ctx) return PMP_EINVAL;
     int64_t b = 0;
-    for (auto& kv : ctx->images) b += (int64_t)kv.second.bytes;
+    for (auto& kv : ctx->images) b += (int64_t)kv.second.bytes * (kv.second.quads ? 5 : 1);
     if (count) *count = (int64_t)ctx->images.size();
     if (bytes) *bytes = b;
     return PMP_OK;

@@ -254,10 +254,11 @@ __global__ __launch_bounds__(kScanBlock) void k_prune_emit(FuseArgs a, const flo
 }
 
 static int geo_args(pmp_ctx* ctx, int W, int H, GeoArgs& g) {
-    int rc = ensure_tables(ctx, W, H, ctx->wc_sigma > 0 ? ctx->wc_sigma : 50.0f);
+    Tables tb;
+    int rc = get_tables(ctx, W, H, ctx->last_sigma, tb);
     if (rc) return rc;
     g.W = W; g.H = H;
-    g.rowtab = ctx->rowtab; g.coltab = ctx->coltab;
+    g.rowtab = tb.row; g.coltab = tb.col;
     g.ku = (float)((double)W / (2.0 * M_PI));
     g.hu = (float)(0.5 * W);
     g.kv = (float)((double)H / M_PI);

@@ -21,7 +21,7 @@ struct KfArgs {
     int W, H, N, half, stride, T;
     int agg_mode, topk, edge_mode, coverage, init_mode, n_refine;
     const uint8_t* ref;                   // H*W
-    const uint32_t* quads;                // N*H*W packed 2x2 neighbourhoods, view v at v*H*W
+    const uint32_t* vq[kMaxViews];        // per view: H*W packed 2x2 neighbourhood words
     const float2* rowtab;                 // H: (cos lat, sin lat)
     const float2* coltab;                 // W: (sin lon, cos lon)
     const float* Wc;                      // 256: exp(-a^2 / (2 sigma_color^2))
@@ -30,7 +30,7 @@ struct KfArgs {
     float c[kMaxViews][3];                // t_n - M t_ref
     float ku, hu, kv, hv;                 // u = lon*ku + hu, v = asin*kv + hv
     float umax, vmax;                     // sample clamps: W (wrap) or W-1, and H-1
-    uint32_t quads_bytes;                 // size of the packed-view buffer
+    uint32_t view_bytes;                  // bytes of one packed view (4*H*W)
     float var_floor, fold_agree, inv_k;
     float depth_penalty, plane_penalty;
     float dr[kMaxRefine], nr[kMaxRefine];
@@ -38,11 +38,15 @@ struct KfArgs {
     uint32_t key0, key1;
 };
 
-// Maps of one keyframe workspace (device).
-struct Maps {
-    float4* plane = nullptr;
-    float* depth = nullptr;
-    float* score = nullptr;
+// Device maps of one keyframe (grown on demand, kept).
+struct SlotBuf {
+    size_t cap = 0;
+    int W = 0, H = 0;
+    float4 *plane = nullptr, *iplane = nullptr, *oplane = nullptr;   // working, after init, final
+    float *depth = nullptr, *score = nullptr, *idepth = nullptr, *iscore = nullptr;
+    float *odepth = nullptr, *oscore = nullptr, *tmp0 = nullptr, *tmp1 = nullptr;
+    int32_t* onghbr = nullptr;
+    static constexpr size_t kBytesPerPixel = 3 * 16 + 9 * 4;
 };
 
 template <typename T>
@@ -55,8 +59,13 @@ struct Image {
     int W = 0, H = 0;
     uint8_t* gray = nullptr;          // H*W
     size_t bytes = 0;
-    uint64_t generation = 0;          // changes whenever the id is re-uploaded
+    uint32_t* quads = nullptr;        // H*W words: (x,y) (x+1,y) (x,y+1) (x+1,y+1), built on first use
+    int quads_wrap = -1;              // column x+1 of the last column: 1 wraps to 0, 0 clamps; -1 not built
+    uint64_t last_use = 0;            // pool clock of the last request that read it (LRU)
+    int pins = 0;                     // requests in flight that read it (never evicted while > 0)
 };
+
+struct QueueState;   // pmp_queue.h
 
 struct PrunerView {
     int W = 0, H = 0;
@@ -83,25 +92,14 @@ struct pmp_ctx {
     std::vector<double> view_poses;   // N*12
     int W = 0, H = 0;
 
-    // workspace (grown on demand, never freed per call)
-    size_t ws_pixels = 0;
-    pmp::Maps work, init;
-    float4* out_plane = nullptr;
-    float* out_depth = nullptr;
-    float* out_score = nullptr;
-    int32_t* out_nghbr = nullptr;
-    uint32_t* views_q = nullptr;      // packed neighbours of the current keyframe, contiguous
-    size_t views_q_cap = 0;           // in words
-    std::vector<uint64_t> packed_key; // (id, generation) pairs + edge flag of views_q
-    uint64_t next_generation = 1;
-    float* tmp0 = nullptr;
-    float* tmp1 = nullptr;
-    uint8_t* mask = nullptr;
-    float2* rowtab = nullptr;
-    float2* coltab = nullptr;
-    int tab_W = 0, tab_H = 0;
-    float* wc = nullptr;
-    float wc_sigma = -1.0f;
+    // workspace of the one-keyframe path (grown on demand, never freed per call)
+    pmp::SlotBuf ws;
+    uint64_t pool_clock = 0;          // advances once per request that reads the pool
+    uint8_t* mask = nullptr;          // staging of a host mask (ws.cap bytes)
+    std::unordered_map<int, float2*> rowtabs;   // by H: (cos lat, sin lat)
+    std::unordered_map<int, float2*> coltabs;   // by W: (sin lon, cos lon)
+    std::unordered_map<uint32_t, float*> wcs;   // by the bits of sigma_color: exp(-a^2 / (2 sigma^2)), a < 256
+    float last_sigma = 50.0f;                   // sigma_color of the last estimator call (used by the Cleaner)
     unsigned long long* counters = nullptr;   // [0] evals, [1] candidates
     bool tables_ok = false;
 
@@ -129,6 +127,9 @@ struct pmp_ctx {
     size_t events_used = 0;           // pairs recorded
     int densify_calls = 0;
 
+    // descriptors, launch configuration and graphs of the work-queue kernels
+    pmp::QueueState* queue = nullptr;
+
     // debug
     float* dbg = nullptr;
     size_t dbg_cap = 0;
@@ -143,9 +144,25 @@ int timing_begin(pmp_ctx* ctx, int stage);
 void timing_end(pmp_ctx* ctx, int pair);
 int check_hip(pmp_ctx* ctx, hipError_t e, const char* what);
 int ensure_workspace(pmp_ctx* ctx, int W, int H);
-int ensure_tables(pmp_ctx* ctx, int W, int H, float sigma_color);
+struct Tables {
+    const float2* row = nullptr;
+    const float2* col = nullptr;
+    const float* wc = nullptr;
+};
+// Bearing tables of a W x H image and the colour-weight table of sigma_color
+// (built once per size / sigma, kept for the context's lifetime).
+int get_tables(pmp_ctx* ctx, int W, int H, float sigma_color, Tables& t);
+// Packed 2x2 words of a pooled image for the given column wrap (built on
+// first use, on `stream`).
+int ensure_quads(pmp_ctx* ctx, Image& im, int wrap, hipStream_t stream);
 void host_tables(int W, int H, std::vector<float2>& rowtab, std::vector<float2>& coltab);
 void derive_poses(const double* ref_pose, int N, const double* view_poses, float (*M)[9], float (*c)[3]);
+int check_params(pmp_ctx* ctx, const pmp_params* p);
+void queue_free(pmp_ctx* ctx);
+int ensure_slot(pmp_ctx* ctx, SlotBuf& b, int W, int H);
+void free_slot(SlotBuf& b);
+int build_args(pmp_ctx* ctx, const pmp_params* p, uint64_t ref_id, int N, const uint64_t* view_ids,
+               const double* ref_pose, const double* view_poses, hipStream_t stream, KfArgs& a);
 
 }  // namespace pmp
 

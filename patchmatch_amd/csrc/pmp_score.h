@@ -98,14 +98,14 @@ PMP_DF float ncc(const RefStat& rs, float floor_, float Swy, float Swyy, float S
 }
 
 // Neighbour sample at (u, v); texel k at integer k (PatchMatchCuda.cu:146 looks
-// up (u+0.5, v+0.5) with texel centres at k+0.5).  The packed words of all
-// views are read through one buffer resource with 32-bit byte offsets
-// (view v starts at word v*W*H; offsets past the buffer read 0).
+// up (u+0.5, v+0.5) with texel centres at k+0.5).  The packed words of a view
+// are read with a uniform 64-bit base and a 32-bit byte offset; the clamps
+// below keep every offset inside the view (a NaN coordinate clamps to 0).
 //   kZero: CPU semantics (depthmap.cc:36-38), 0 outside [0,W-1) x [0,H-1);
 //   else:  u clamped to [0, umax] (umax = W with longitude wrap packed into
 //          the words, W-1 for clamp), v clamped to [0, H-1].
 template <bool kZero>
-PMP_DF float sample(const KfArgs& a, __amdgpu_buffer_rsrc_t rsrc, uint32_t view_off, float u, float v) {
+PMP_DF float sample(const KfArgs& a, const uint32_t* __restrict__ base, float u, float v) {
     const int W = a.W;
     float uu, vv;
     bool inside = true;
@@ -121,8 +121,8 @@ PMP_DF float sample(const KfArgs& a, __amdgpu_buffer_rsrc_t rsrc, uint32_t view_
     const float fx = uu - x0f, fy = vv - y0f;
     int x0 = (int)x0f;
     if constexpr (!kZero) x0 = (x0 >= W) ? x0 - W : x0;
-    const uint32_t idx = view_off + (uint32_t)((int)y0f * W + x0);
-    const uint32_t word = __builtin_amdgcn_raw_buffer_load_b32(rsrc, idx * 4u, 0, 0);
+    const uint32_t idx = (uint32_t)((int)y0f * W + x0);
+    const uint32_t word = *(const uint32_t*)((const char*)base + idx * 4u);
     const float b0 = (float)(word & 0xFFu);
     const float b1 = (float)((word >> 8) & 0xFFu);
     const float b2 = (float)((word >> 16) & 0xFFu);
@@ -132,10 +132,6 @@ PMP_DF float sample(const KfArgs& a, __amdgpu_buffer_rsrc_t rsrc, uint32_t view_
     const float val = fma_(fy, bot - top, top);
     if constexpr (kZero) return inside ? val : 0.0f;
     return val;
-}
-
-PMP_DF __amdgpu_buffer_rsrc_t quad_rsrc(const KfArgs& a) {
-    return __builtin_amdgcn_make_buffer_rsrc((void*)a.quads, (short)0, (int)a.quads_bytes, 0x00020000);
 }
 
 // Equirectangular projection (PatchMatchCuda.cu:132-144).
@@ -185,14 +181,62 @@ PMP_DF float aggregate(const KfArgs& a, float (&sv)[NV]) {
     return s;
 }
 
-// Moments of views [V0, V0+G) (G <= 3 live views keep the per-view
-// transforms in scalar registers).  Per view the taps are accumulated in tap
-// order, exactly as in a single pass over all views.
-template <int NV, int V0, int G, bool kZero>
+// Aggregation of N (run-time) per-view scores kept in LDS at sv[v * ld]
+// (v < N): the value of aggregate<NV> over the same scores (REF_FOLD and MAX
+// are the same sequential folds; for TOPK_MEAN each score is inserted into a
+// descending list of the topk largest, which are the first topk positions of
+// aggregate's exchange sort, and summed in the same order).  Per-view scores
+// are finite (ncc clamps to [-1, 1]).
+PMP_DF float aggregate_rt(const KfArgs& a, const float* sv, int ld) {
+    const int N = a.N;
+    if (a.agg_mode == PMP_AGG_TOPK_MEAN) {
+        float top[kMaxViews];
+#pragma unroll
+        for (int p = 0; p < kMaxViews; ++p) top[p] = -3.0e38f;
+        for (int v = 0; v < N; ++v) {
+            float x = sv[v * ld];
+#pragma unroll
+            for (int p = 0; p < kMaxViews; ++p) {
+                if (p < a.topk) {
+                    const float hi = (x > top[p]) ? x : top[p];
+                    const float lo = (x > top[p]) ? top[p] : x;
+                    top[p] = hi;
+                    x = lo;
+                }
+            }
+        }
+        float sum = top[0];
+#pragma unroll
+        for (int p = 1; p < kMaxViews; ++p)
+            if (p < a.topk) sum = sum + top[p];
+        return sum * a.inv_k;
+    }
+    float s = -1.0f;
+    if (a.agg_mode == PMP_AGG_MAX) {
+        for (int v = 0; v < N; ++v) {
+            const float s2 = sv[v * ld];
+            s = (s2 > s) ? s2 : s;
+        }
+        return s;
+    }
+    for (int v = 0; v < N; ++v) {
+        const float s2 = sv[v * ld];
+        if (abs_(s - s2) < a.fold_agree * abs_(s2)) s = 0.5f * (s + s2);
+        else s = (s2 > s) ? s2 : s;
+    }
+    return s;
+}
+
+// Moments of views [v0, v0+G) (G <= 3 live views keep the per-view
+// transforms in scalar registers; v0 is uniform).  Per view the taps are
+// accumulated in tap order, exactly as in a single pass over all views.
+template <int G, bool kZero>
 PMP_DF void view_group(const KfArgs& a, const float* sWc, const RefStat& rs, int i, int j, float px, float py,
-                       float pz, float (&sv)[NV]) {
-    const int W = a.W, H = a.H;
-    const __amdgpu_buffer_rsrc_t rsrc = quad_rsrc(a);
+                       float pz, int v0, float (&out)[G]) {
+    const int W = a.W;
+    const uint32_t* base[G];
+#pragma unroll
+    for (int g = 0; g < G; ++g) base[g] = a.vq[v0 + g];
     float acc0[G], acc1[G], acc2[G], shift[G];
 #pragma unroll
     for (int g = 0; g < G; ++g) { acc0[g] = 0.0f; acc1[g] = 0.0f; acc2[g] = 0.0f; shift[g] = 0.0f; }
@@ -213,13 +257,13 @@ PMP_DF void view_group(const KfArgs& a, const float* sWc, const RefStat& rs, int
             const float wxc = w * (xs - rs.mx);
 #pragma unroll
             for (int g = 0; g < G; ++g) {
-                const int v = V0 + g;
+                const int v = v0 + g;
                 const float x = fma_(a.M[v][0], xc, fma_(a.M[v][1], yc, fma_(a.M[v][2], zc, a.c[v][0])));
                 const float y = fma_(a.M[v][3], xc, fma_(a.M[v][4], yc, fma_(a.M[v][5], zc, a.c[v][1])));
                 const float z = fma_(a.M[v][6], xc, fma_(a.M[v][7], yc, fma_(a.M[v][8], zc, a.c[v][2])));
                 float u, vv;
                 project(a, x, y, z, u, vv);
-                const float val = sample<kZero>(a, rsrc, (uint32_t)(v * W * H), u, vv);
+                const float val = sample<kZero>(a, base[g], u, vv);
                 shift[g] = (t == 0) ? val : shift[g];
                 const float yd = val - shift[g];
                 const float tw = w * yd;
@@ -230,7 +274,7 @@ PMP_DF void view_group(const KfArgs& a, const float* sWc, const RefStat& rs, int
         }
     }
 #pragma unroll
-    for (int g = 0; g < G; ++g) sv[V0 + g] = ncc(rs, a.var_floor, acc0[g], acc1[g], acc2[g]);
+    for (int g = 0; g < G; ++g) out[g] = ncc(rs, a.var_floor, acc0[g], acc1[g], acc2[g]);
 }
 
 template <int NV, int V0, bool kZero>
@@ -238,18 +282,48 @@ PMP_DF void view_groups(const KfArgs& a, const float* sWc, const RefStat& rs, in
                         float pz, float (&sv)[NV]) {
     if constexpr (V0 < NV) {
         constexpr int G = (NV - V0) < 3 ? (NV - V0) : 3;
-        view_group<NV, V0, G, kZero>(a, sWc, rs, i, j, px, py, pz, sv);
+        float out[G];
+        view_group<G, kZero>(a, sWc, rs, i, j, px, py, pz, V0, out);
+#pragma unroll
+        for (int g = 0; g < G; ++g) sv[V0 + g] = out[g];
         view_groups<NV, V0 + G, kZero>(a, sWc, rs, i, j, px, py, pz, sv);
     }
 }
 
-// Score of plane (px, py, pz) at pixel (i, j); kZero selects PMP_EDGE_ZERO sampling.
+// Score of plane (px, py, pz) at pixel (i, j); kZero selects PMP_EDGE_ZERO
+// sampling.  NV > 0: compile-time view count (sv unused); NV == 0: a.N views
+// (uniform), in groups of 3, 2 or 1, with the per-view scores parked in LDS
+// at sv[v * ld] until they are aggregated.
 template <int NV, bool kZero>
 PMP_DF float score_plane(const KfArgs& a, const float* sWc, const RefStat& rs, int i, int j, float px, float py,
-                         float pz) {
-    float sv[NV];
-    view_groups<NV, 0, kZero>(a, sWc, rs, i, j, px, py, pz, sv);
-    return aggregate<NV>(a, sv);
+                         float pz, float* sv = nullptr, int ld = 0) {
+    if constexpr (NV > 0) {
+        float s[NV];
+        view_groups<NV, 0, kZero>(a, sWc, rs, i, j, px, py, pz, s);
+        return aggregate<NV>(a, s);
+    } else {
+        const int N = a.N;
+        for (int v0 = 0; v0 < N; v0 += 3) {
+            const int G = (N - v0) < 3 ? (N - v0) : 3;
+            if (G == 3) {
+                float out[3];
+                view_group<3, kZero>(a, sWc, rs, i, j, px, py, pz, v0, out);
+                sv[v0 * ld] = out[0];
+                sv[(v0 + 1) * ld] = out[1];
+                sv[(v0 + 2) * ld] = out[2];
+            } else if (G == 2) {
+                float out[2];
+                view_group<2, kZero>(a, sWc, rs, i, j, px, py, pz, v0, out);
+                sv[v0 * ld] = out[0];
+                sv[(v0 + 1) * ld] = out[1];
+            } else {
+                float out[1];
+                view_group<1, kZero>(a, sWc, rs, i, j, px, py, pz, v0, out);
+                sv[v0 * ld] = out[0];
+            }
+        }
+        return aggregate_rt(a, sv, ld);
+    }
 }
 
 // PlaneFromDepthNormal (PatchMatchCuda.cu:31-43)

@@ -1,0 +1,234 @@
+"""Batched and served workloads of the benchmark (synthetic, seeded).
+
+Both draw keyframes from seeded trajectories through the config-1 scene
+(room + 20 spheres + 1.2 m tube), one trajectory per resolution class,
+rendered on the GPU (synth_gpu).  A keyframe's neighbours are its nearest
+trajectory neighbours (0.2 m spacing), so consecutive requests share most of
+their images, as a SLAM keyframe stream does.
+
+batched: n keyframes with resolutions drawn from `classes` and view counts
+    U{2..10}; depth maps/s of the batched form (one work-queue launch per
+    stage over all keyframes, hipGraph, autotuned launch shape) against the
+    plain form (pmp_densify per keyframe, images pooled) on the same set.
+served: a Poisson stream of requests at a given rate; the served form
+    (pmp_server, continuous batching, LRU pool) against the plain
+    per-request path (upload the request's images, pmp_densify, read the
+    maps back; one request at a time in arrival order) on the same stream.
+    Latency is measured from a request's arrival to its maps on the host.
+"""
+from __future__ import annotations
+
+import threading
+import time
+
+import numpy as np
+
+BATCH_CLASSES = [(960, 480, 0.3), (1920, 960, 0.5), (2880, 1440, 0.2)]
+SERVE_CLASSES = [(960, 480, 0.3), (1920, 960, 0.4), (2880, 1440, 0.2), (5760, 2880, 0.1)]
+
+
+class Stream:
+    """Rendered trajectories per resolution class and request specs over them."""
+
+    def __init__(self, seed: int, classes, per_class: int = 16, spacing: float = 0.2):
+        from . import synth_gpu
+        self.classes = classes
+        self.seqs = []
+        for c, (W, H, _) in enumerate(classes):
+            self.seqs.append(synth_gpu.sequence(seed * 31 + c, per_class, W, H, spacing))
+        self.per_class = per_class
+
+    def image_id(self, c, k):
+        return 1_000_000 * (c + 1) + k
+
+    def mask_id(self, c):
+        return 1_000_000 * (c + 1) + 999_999
+
+    def draw(self, rng, n):
+        """n request specs: (class, ref index, neighbour indices)."""
+        w = np.array([c[2] for c in self.classes], dtype=np.float64)
+        cls = rng.choice(len(self.classes), size=n, p=w / w.sum())
+        out = []
+        for c in cls:
+            k = int(rng.integers(0, self.per_class))
+            nv = int(rng.integers(2, 11))
+            others = sorted((m for m in range(self.per_class) if m != k), key=lambda m: (abs(m - k), m))
+            out.append((int(c), k, others[:nv]))
+        return out
+
+    def keyframe(self, spec):
+        c, k, nb = spec
+        s = self.seqs[c]
+        return dict(ref_id=self.image_id(c, k), ref_pose=s["poses"][k], ref=s["images"][k],
+                    view_ids=[self.image_id(c, m) for m in nb], view_poses=[s["poses"][m] for m in nb],
+                    views=[s["images"][m] for m in nb], mask_id=self.mask_id(c), mask=s["mask"],
+                    gt=s["gt"][k], shape=s["images"][k].shape)
+
+
+def accuracy(depth, gt, mask):
+    unmasked = mask > 0
+    valid = unmasked & (depth > 0)
+    rel = np.abs(depth.astype(np.float64) - gt) / np.maximum(gt, 1e-9)
+    return dict(valid_fraction=float(valid.sum() / max(unmasked.sum(), 1)),
+                absrel_le_1pct_of_valid=float(np.mean(rel[valid] <= 0.01)) if valid.any() else 0.0,
+                median_absrel=float(np.median(rel[valid])) if valid.any() else None)
+
+
+def run_batched(stream: Stream, params, n: int, seed: int):
+    from ._lib import PMP_MAPS_FINAL, Context, Keyframe
+    rng = np.random.default_rng(seed)
+    specs = stream.draw(rng, n)
+    ctx = Context(0)
+    descs, kfs = [], []
+    put = set()
+    for sp in specs:
+        kf = stream.keyframe(sp)
+        for iid, img in [(kf["ref_id"], kf["ref"])] + list(zip(kf["view_ids"], kf["views"])) + \
+                [(kf["mask_id"], kf["mask"])]:
+            if iid not in put:
+                ctx.image_put(iid, img)
+                put.add(iid)
+        kfs.append(kf)
+        descs.append(Keyframe.make(kf["ref_id"], kf["ref_pose"], kf["view_ids"], kf["view_poses"],
+                                   mask_id=kf["mask_id"]))
+    pix = sum(kf["shape"][0] * kf["shape"][1] for kf in kfs)
+    # plain: one keyframe per call, images already pooled
+    ctx.set_views(kfs[0]["ref_id"], kfs[0]["ref_pose"], kfs[0]["view_ids"], kfs[0]["view_poses"])
+    ctx.densify(params)
+    ctx.synchronize()
+    t = time.perf_counter()
+    for kf in kfs:
+        ctx.set_views(kf["ref_id"], kf["ref_pose"], kf["view_ids"], kf["view_poses"])
+        ctx.set_mask(kf["mask_id"])
+        ctx.densify(params)
+    ctx.synchronize()
+    t_plain = time.perf_counter() - t
+    ctx.H, ctx.W = kfs[-1]["shape"]
+    last_plain = ctx.get_maps(PMP_MAPS_FINAL)
+    # batched: autotune the launch shape on this batch, then time a densification
+    cfg = ctx.batch_autotune(params, descs)
+    ctx.batch_densify(params, descs)
+    ctx.synchronize()
+    t = time.perf_counter()
+    ctx.batch_densify(params, descs)
+    ctx.synchronize()
+    t_batch = time.perf_counter() - t
+    last_batch = ctx.batch_get_maps(n - 1, kfs[-1]["shape"])
+    same = all(np.array_equal(np.ascontiguousarray(a).view(np.uint32), np.ascontiguousarray(b).view(np.uint32))
+               for a, b in zip(last_plain, last_batch))
+    acc = [accuracy(ctx.batch_get_maps(k, kfs[k]["shape"])[1], kfs[k]["gt"], kfs[k]["mask"]) for k in range(n)]
+    ctx.close()
+    res = {}
+    for W, H, _ in stream.classes:
+        res[f"{W}x{H}"] = sum(1 for kf in kfs if kf["shape"] == (H, W))
+    return {
+        "keyframes": n, "megapixels": round(pix / 1e6, 2), "resolutions": res,
+        "views_mean": round(float(np.mean([len(kf["view_ids"]) for kf in kfs])), 2),
+        "plain_depth_maps_per_s": round(n / t_plain, 3), "batched_depth_maps_per_s": round(n / t_batch, 3),
+        "speedup": round(t_plain / t_batch, 3), "plain_s": round(t_plain, 3), "batched_s": round(t_batch, 3),
+        "autotuned_config": cfg, "last_keyframe_bit_identical": bool(same),
+        "accuracy_mean": {k: round(float(np.mean([a[k] for a in acc if a[k] is not None])), 4)
+                          for k in ("valid_fraction", "absrel_le_1pct_of_valid", "median_absrel")},
+    }
+
+
+def _arrivals(rng, rate, n):
+    return np.cumsum(rng.exponential(1.0 / rate, n))
+
+
+def _summary(lat, t_first, t_last, n):
+    lat = np.asarray(lat) * 1e3
+    return {"throughput_per_s": round(n / (t_last - t_first), 3), "p50_ms": round(float(np.percentile(lat, 50)), 1),
+            "p99_ms": round(float(np.percentile(lat, 99)), 1), "max_ms": round(float(lat.max()), 1)}
+
+
+def run_plain_stream(stream: Stream, params, specs, arrivals):
+    """One request at a time in arrival order: upload its images, densify, read back."""
+    from ._lib import PMP_MAPS_FINAL, Context
+    ctx = Context(0)
+    # warm the kernels once (not timed)
+    kf = stream.keyframe(specs[0])
+    ctx.load_keyframe(kf["ref"], kf["views"], kf["ref_pose"], kf["view_poses"])
+    ctx.densify(params, kf["mask"])
+    ctx.synchronize()
+    lat, done = [], []
+    t0 = time.perf_counter()
+    for spec, ta in zip(specs, arrivals):
+        now = time.perf_counter() - t0
+        if now < ta:
+            time.sleep(ta - now)
+        kf = stream.keyframe(spec)
+        ctx.load_keyframe(kf["ref"], kf["views"], kf["ref_pose"], kf["view_poses"])
+        ctx.densify(params, kf["mask"])
+        ctx.get_maps(PMP_MAPS_FINAL)
+        t = time.perf_counter() - t0
+        lat.append(t - ta)
+        done.append(t)
+    ctx.close()
+    return _summary(lat, arrivals[0], max(done), len(specs))
+
+
+def run_served_stream(stream: Stream, params, specs, arrivals, max_slots=16, hbm_budget=64 << 30):
+    """Requests submitted at their arrival times to a pmp_server."""
+    from ._lib import Server
+    srv = Server(0, max_slots=max_slots, hbm_budget=hbm_budget, edge_mode=params.edge_mode)
+    # warm the kernels once (not timed)
+    kf = stream.keyframe(specs[0])
+    t = srv.submit(params, kf["ref_id"], kf["ref_pose"], kf["view_ids"], kf["view_poses"], kf["shape"],
+                   ref=kf["ref"], views=kf["views"], mask_id=kf["mask_id"], mask=kf["mask"])
+    srv.wait(t)
+    srv.fetch(t)
+    tickets = [None] * len(specs)
+    t_py0 = time.perf_counter()
+    base = {}
+
+    def producer():
+        for q, (spec, ta) in enumerate(zip(specs, arrivals)):
+            now = time.perf_counter() - t_py0
+            if now < ta:
+                time.sleep(ta - now)
+            kf = stream.keyframe(spec)
+            tickets[q] = srv.submit(params, kf["ref_id"], kf["ref_pose"], kf["view_ids"], kf["view_poses"],
+                                    kf["shape"], ref=kf["ref"], views=kf["views"], mask_id=kf["mask_id"],
+                                    mask=kf["mask"])
+            base.setdefault("first", time.perf_counter())
+
+    th = threading.Thread(target=producer)
+    th.start()
+    lat, done = [], []
+    infos = []
+    for q in range(len(specs)):
+        while tickets[q] is None:
+            time.sleep(0.0005)
+        srv.wait(tickets[q])
+        *_, info = srv.fetch(tickets[q])
+        infos.append(info)
+    th.join()
+    # server clock -> this clock: the warm-up request's fetch anchors nothing; use the
+    # first submission's own submit stamp against the time the producer recorded
+    t_srv0 = infos[0]["submit_ms"] / 1e3
+    off = base["first"] - t_py0 - t_srv0      # py time (relative to t_py0) = srv time + off
+    for info, ta in zip(infos, arrivals):
+        d = info["done_ms"] / 1e3 + off
+        lat.append(d - ta)
+        done.append(d)
+    st = srv.stats()
+    srv.close()
+    out = _summary(lat, arrivals[0], max(done), len(specs))
+    out.update({"max_in_flight": st["max_in_flight"], "uploads": st["uploads"], "evictions": st["evictions"],
+                "steps": st["steps"]})
+    return out
+
+
+def run_served(stream: Stream, params, rates, n_requests: int, seed: int):
+    out = {}
+    for rate in rates:
+        rng = np.random.default_rng(seed + int(rate * 1000))
+        specs = stream.draw(rng, n_requests)
+        arr = _arrivals(rng, rate, n_requests)
+        plain = run_plain_stream(stream, params, specs, arr)
+        served = run_served_stream(stream, params, specs, arr)
+        out[f"{rate:g}_per_s"] = {"plain": plain, "served": served,
+                                  "throughput_ratio": round(served["throughput_per_s"] / plain["throughput_per_s"], 3),
+                                  "p99_ratio": round(served["p99_ms"] / plain["p99_ms"], 3)}
+    return out

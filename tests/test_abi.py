@@ -59,3 +59,32 @@ def test_null_arguments_rejected():
     assert L.pmp_create(None, 0) == -1
     assert L.pmp_synchronize(None) == -1
     assert L.pmp_last_error(None) == b"null context"
+
+
+def test_batch_and_server_struct_layout(tmp_path):
+    """The batch and server structs as a C compiler lays them out from the
+    header equal the ctypes mirrors."""
+    import subprocess
+    fields = [("sizeof(pmp_keyframe)", ctypes.sizeof(_lib.Keyframe)),
+              ("offsetof(pmp_keyframe, n_views)", _lib.Keyframe.n_views.offset),
+              ("offsetof(pmp_keyframe, view_ids)", _lib.Keyframe.view_ids.offset),
+              ("offsetof(pmp_keyframe, view_poses)", _lib.Keyframe.view_poses.offset),
+              ("offsetof(pmp_keyframe, mask_id)", _lib.Keyframe.mask_id.offset),
+              ("offsetof(pmp_keyframe, max_depth)", _lib.Keyframe.max_depth.offset),
+              ("sizeof(pmp_batch_config)", ctypes.sizeof(_lib.BatchConfig)),
+              ("sizeof(pmp_server_opts)", ctypes.sizeof(_lib.ServerOpts)),
+              ("offsetof(pmp_server_opts, config)", _lib.ServerOpts.config.offset),
+              ("sizeof(pmp_request)", ctypes.sizeof(_lib.Request)),
+              ("offsetof(pmp_request, ref_pixels)", _lib.Request.ref_pixels.offset),
+              ("offsetof(pmp_request, mask_pixels)", _lib.Request.mask_pixels.offset),
+              ("sizeof(pmp_request_info)", ctypes.sizeof(_lib.RequestInfo)),
+              ("sizeof(pmp_server_info)", ctypes.sizeof(_lib.ServerInfo)),
+              ("offsetof(pmp_server_info, in_flight)", _lib.ServerInfo.in_flight.offset)]
+    src = tmp_path / "lay.c"
+    body = "".join(f'printf("%zu\\n", (size_t)({e}));' for e, _ in fields)
+    src.write_text('#include <stddef.h>\n#include <stdio.h>\n#include "pmp/pmp.h"\n'
+                   f'int main(void){{{body}return 0;}}\n')
+    exe = tmp_path / "lay"
+    subprocess.run(["gcc", "-std=c11", f"-I{os.path.join(ROOT, 'include')}", str(src), "-o", str(exe)], check=True)
+    got = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
+    assert got == [w for _, w in fields], list(zip([e for e, _ in fields], got, [w for _, w in fields]))

@@ -1,0 +1,160 @@
+"""Served form (pmp_server_*): requests of mixed sizes, view counts and
+params densified with continuous batching.  Each request's maps must be
+bit-identical to the one-keyframe path alone (pmp_densify) -- whether the
+server runs it alone (one slot) or in the mix with others, submitted at once
+or while others are on the device -- and the image pool must evict by LRU
+within its HBM budget."""
+import threading
+import time
+
+import numpy as np
+import pytest
+
+from patchmatch_amd import synth
+from patchmatch_amd._lib import PMP_MAPS_FINAL, PmpError, Server
+from patchmatch_amd.params import EDGE_CLAMP, config1_params, default_params
+
+from conftest import cached
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(15, 320, 160, 6), (16, 256, 128, 2), (17, 384, 192, 4), (18, 192, 96, 10), (19, 224, 112, 5),
+          (20, 192, 96, 1), (21, 320, 160, 3)]
+
+
+def _keyframes():
+    def make():
+        return [synth.make_keyframe(s, W, H, n, baseline=(0.05, 0.5), spheres=8, tube=True) for s, W, H, n in SHAPES]
+    return cached("server_kfs", make)
+
+
+def _params(k):
+    # every request with its own params: aggregation, iterations, seed, depth range
+    if k % 3 == 0:
+        return config1_params(iterations=2, seed=100 + k)
+    if k % 3 == 1:
+        return default_params(iterations=1, seed=100 + k)
+    return config1_params(iterations=3, seed=100 + k, topk=2, max_depth=9.0)
+
+
+def _ids(k, n):
+    base = 1000 * (k + 1)
+    return base, [base + 1 + v for v in range(n)], base + 999
+
+
+def _plain(gpu_ctx, kfs):
+    def run():
+        out = []
+        for k, kf in enumerate(kfs):
+            ref, views, mask = _ids(k, len(kf.views))
+            gpu_ctx.image_put(ref, kf.ref)
+            for v, img in enumerate(kf.views):
+                gpu_ctx.image_put(views[v], img)
+            gpu_ctx.set_views(ref, kf.ref_pose, views, kf.view_poses)
+            gpu_ctx.H, gpu_ctx.W = kf.H, kf.W
+            gpu_ctx.densify(_params(k), kf.mask)
+            out.append(gpu_ctx.get_maps(PMP_MAPS_FINAL))
+        return out
+    return cached("server_plain", run)
+
+
+def _submit(srv, kfs, k, with_pixels=True):
+    kf = kfs[k]
+    ref, views, mask = _ids(k, len(kf.views))
+    return srv.submit(_params(k), ref, kf.ref_pose, views, kf.view_poses, (kf.H, kf.W),
+                      ref=kf.ref if with_pixels else None, views=kf.views if with_pixels else None,
+                      mask_id=mask, mask=kf.mask if with_pixels else None)
+
+
+def _check(got, want, tag):
+    for name, u, v in zip(("plane", "depth", "score", "nghbr"), got[:4], want):
+        same = np.ascontiguousarray(u).view(np.uint32) == np.ascontiguousarray(v).view(np.uint32)
+        assert same.all(), f"{tag}: {name}: {np.count_nonzero(~same)} of {same.size} differ"
+
+
+@pytest.mark.parametrize("slots", [1, 4, 16])
+def test_server_same_alone_and_in_the_mix(gpu_ctx, slots):
+    kfs = _keyframes()
+    want = _plain(gpu_ctx, kfs)
+    print(f"[server slots={slots}] {len(kfs)} requests submitted at once", flush=True)
+    with Server(0, max_slots=slots) as srv:
+        srv.pause(True)          # queue everything, then admit as many as fit at once
+        tickets = [_submit(srv, kfs, k) for k in range(len(kfs))]
+        assert srv.stats()["queued"] == len(kfs)
+        srv.pause(False)
+        for k, t in enumerate(tickets):
+            assert srv.wait(t, 60000), f"request {k} not done in 60 s"
+            got = srv.fetch(t)
+            _check(got, want[k], f"slots={slots} request {k}")
+            assert got[4]["steps"] == 2 * _params(k).iterations + 2
+        st = srv.stats()
+        assert st["completed"] == len(kfs) and st["failed"] == 0
+        assert st["max_in_flight"] == min(slots, len(kfs))
+
+
+def test_server_staggered_arrivals(gpu_ctx):
+    """Requests submitted from another thread while earlier ones run."""
+    kfs = _keyframes()
+    want = _plain(gpu_ctx, kfs)
+    with Server(0, max_slots=3) as srv:
+        tickets = {}
+
+        def producer():
+            for k in range(len(kfs)):
+                tickets[k] = _submit(srv, kfs, k)
+                time.sleep(0.003 * (k % 3))
+        th = threading.Thread(target=producer)
+        th.start()
+        th.join(60)
+        assert len(tickets) == len(kfs)
+        for k in range(len(kfs)):
+            assert srv.wait(tickets[k], 60000)
+            _check(srv.fetch(tickets[k]), want[k], f"staggered request {k}")
+        print(f"[server staggered] {srv.stats()}", flush=True)
+
+
+def test_server_pool_lru_and_budget(gpu_ctx):
+    """A budget that holds the maps of one slot plus a few images: images are
+    evicted least-recently-used first, pooled ids are reused without pixels,
+    and evicted ids must be given pixels again."""
+    kfs = _keyframes()
+    want = _plain(gpu_ctx, kfs)
+    k0 = 0   # 320x160, 6 views
+    kf = kfs[k0]
+    P = kf.W * kf.H
+    budget = P * 84 + P * (1 + 5 * 6 + 1) + 3 * P * 5
+    with Server(0, max_slots=1, hbm_budget=budget) as srv:
+        t = _submit(srv, kfs, k0)
+        assert srv.wait(t, 60000)
+        _check(srv.fetch(t), want[k0], "first")
+        # pooled: no pixels needed, no uploads
+        up = srv.stats()["uploads"]
+        t = _submit(srv, kfs, k0, with_pixels=False)
+        assert srv.wait(t, 60000)
+        _check(srv.fetch(t), want[k0], "pooled")
+        assert srv.stats()["uploads"] == up
+        # another keyframe of the same size does not fit next to the first: evictions
+        k6 = 6   # 320x160, 3 views
+        t = _submit(srv, kfs, k6)
+        assert srv.wait(t, 60000)
+        _check(srv.fetch(t), want[k6], "after eviction")
+        st = srv.stats()
+        assert st["evictions"] > 0 and st["pool_bytes"] + st["maps_bytes"] <= budget
+        # the first keyframe's reference was evicted (oldest): pixels are required again
+        with pytest.raises(PmpError, match="ENOTFOUND"):
+            _submit(srv, kfs, k0, with_pixels=False)
+
+
+def test_server_rejects_bad_requests(gpu_ctx):
+    kfs = _keyframes()
+    with Server(0, max_slots=2) as srv:
+        kf = kfs[1]
+        ref, views, mask = _ids(1, len(kf.views))
+        with pytest.raises(PmpError, match="EINVAL"):
+            srv.submit(_params(1).copy(edge_mode=EDGE_CLAMP), ref, kf.ref_pose, views, kf.view_poses, (kf.H, kf.W),
+                       ref=kf.ref, views=kf.views)
+        with pytest.raises(PmpError, match="ENOMEM"):
+            Server(0, max_slots=1, hbm_budget=1000).submit(_params(1), ref, kf.ref_pose, views, kf.view_poses,
+                                                          (kf.H, kf.W), ref=kf.ref, views=kf.views)
+        with pytest.raises(PmpError, match="ENOTFOUND"):
+            srv.wait(987654, 0)
