@@ -55,6 +55,22 @@ def build_hip(verbose: bool = False) -> str:
     return LIB
 
 
+SYNTH_LIB = os.path.join(HERE, "libpmpsynth.so")
+
+
+def build_synth(verbose: bool = False) -> str:
+    """Benchmark-input ray caster (csrc/synth_render.hip -> libpmpsynth.so)."""
+    tmp = SYNTH_LIB + ".tmp"
+    cmd = [HIPCC, *FLAGS, "-shared", os.path.join(CSRC, "synth_render.hip"), "-o", tmp]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"hipcc failed for synth_render.hip:\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, SYNTH_LIB)
+    if verbose:
+        print(f"built {SYNTH_LIB}")
+    return SYNTH_LIB
+
+
 EXAMPLES = ["facade_window"]
 
 
@@ -86,6 +102,7 @@ def build_oracle(verbose: bool = False) -> str:
 
 if __name__ == "__main__":
     build_hip(verbose=True)
+    build_synth(verbose=True)
     build_examples(verbose=True)
     build_oracle(verbose=True)
     sys.exit(0)

@@ -161,6 +161,46 @@ def render(scene: Scene, pose: np.ndarray, W: int, H: int):
     return img.reshape(H, W), depth.reshape(H, W)
 
 
+_synth_lib = None
+
+
+def render_gpu(scene: Scene, pose: np.ndarray, W: int, H: int):
+    """render() on the GPU (libpmpsynth.so, built from csrc/synth_render.hip):
+    the same cast and texture in double precision; float32 intensity and depth.
+    For benchmark inputs at sizes the numpy path cannot render in seconds."""
+    import ctypes
+    import os
+    global _synth_lib
+    if _synth_lib is None:
+        path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libpmpsynth.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} is not built; run __graft_entry__.build()")
+        L = ctypes.CDLL(path)
+        dp, ip, fp = (ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int32),
+                      ctypes.POINTER(ctypes.c_float))
+        L.pmps_render.argtypes = [dp, ctypes.c_int, ctypes.c_int, dp, ctypes.c_int, dp, ip, dp, ctypes.c_int,
+                                  ctypes.c_double, fp, fp]
+        L.pmps_render.restype = ctypes.c_int
+        _synth_lib = L
+    dp = ctypes.POINTER(ctypes.c_double)
+    pose = np.ascontiguousarray(pose, dtype=np.float64)
+    sph = np.ascontiguousarray(scene.spheres, dtype=np.float64).reshape(-1, 4)
+    tube = None if scene.tube is None else np.ascontiguousarray(scene.tube, dtype=np.float64)
+    perm = np.ascontiguousarray(scene.noise.perm, dtype=np.int32)
+    vals = np.ascontiguousarray(scene.noise.vals, dtype=np.float64)
+    img = np.zeros((H, W), np.float32)
+    dep = np.zeros((H, W), np.float32)
+    rc = _synth_lib.pmps_render(pose.ctypes.data_as(dp), W, H, sph.ctypes.data_as(dp), sph.shape[0],
+                                None if tube is None else tube.ctypes.data_as(dp),
+                                perm.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), vals.ctypes.data_as(dp),
+                                int(scene.noise.mask), float(scene.base_freq),
+                                img.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                dep.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
+    if rc != 0:
+        raise RuntimeError(f"pmps_render failed with HIP error {rc}")
+    return img, dep
+
+
 def photometric(img: np.ndarray, rng: np.random.Generator) -> np.ndarray:
     """gain U(0.8,1.2), bias U(-10,10), N(0, 2^2) noise, clamp, round -> uint8."""
     gain = rng.uniform(0.8, 1.2)
@@ -246,6 +286,25 @@ def config0(seed: int = 0) -> Keyframe:
 def config1(seed: int = 0, W: int = 1920, H: int = 960, n_views: int = 6, workers: int = 8) -> Keyframe:
     """1920x960, reference + 6 neighbours, room + 20 spheres + tube, baselines 0.05-0.5 m."""
     return make_keyframe(seed, W, H, n_views, baseline=(0.05, 0.5), spheres=20, tube=True, workers=workers)
+
+
+def sequence_gpu(seed: int, n: int, W: int, H: int, spacing: float = 0.2, mask_frac: float = 0.15):
+    """n keyframes along the seeded trajectory through the config-1 scene
+    (room + 20 spheres + tube, spheres kept clear of the path), rendered at
+    W x H on the GPU: dict with poses (n, 12), images (n x (H, W) uint8), gt
+    (n x (H, W) float32 depth) and the mask."""
+    s_scene, s_photo = _split_seeds(seed ^ 0x5E0, 2)
+    scene = make_scene(s_scene, spheres=20, tube=True, cam_region=2.7)
+    poses = trajectory(seed, n, spacing)
+    prng = np.random.default_rng(s_photo)
+    images, gts = [], []
+    for k in range(n):
+        img, dep = render_gpu(scene, poses[k], W, H)
+        images.append(photometric(img.astype(np.float64), prng))
+        gts.append(dep)
+    mask = np.ones((H, W), np.uint8)
+    mask[int(np.floor((1.0 - mask_frac) * H)):, :] = 0
+    return {"poses": poses, "images": images, "gt": gts, "mask": mask}
 
 
 def trajectory(seed: int, n: int, spacing: float = 0.2):

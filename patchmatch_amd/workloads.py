@@ -2,7 +2,7 @@
 
 Both draw keyframes from seeded trajectories through the config-1 scene
 (room + 20 spheres + 1.2 m tube), one trajectory per resolution class,
-rendered on the GPU (synth_gpu).  A keyframe's neighbours are its nearest
+rendered on the GPU (synth.sequence_gpu).  A keyframe's neighbours are its nearest
 trajectory neighbours (0.2 m spacing), so consecutive requests share most of
 their images, as a SLAM keyframe stream does.
 
@@ -31,11 +31,11 @@ class Stream:
     """Rendered trajectories per resolution class and request specs over them."""
 
     def __init__(self, seed: int, classes, per_class: int = 16, spacing: float = 0.2):
-        from . import synth_gpu
+        from . import synth
         self.classes = classes
         self.seqs = []
         for c, (W, H, _) in enumerate(classes):
-            self.seqs.append(synth_gpu.sequence(seed * 31 + c, per_class, W, H, spacing))
+            self.seqs.append(synth.sequence_gpu(seed * 31 + c, per_class, W, H, spacing))
         self.per_class = per_class
 
     def image_id(self, c, k):
