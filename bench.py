@@ -36,10 +36,10 @@ sys.path.insert(0, ROOT)
 
 FP32_PEAK_TFLOPS = 157.3       # MI355X published FP32 peak
 HBM_PEAK_GBS = 8000.0
-# L2-to-memory bytes per k_propagate launch on the 1920x960 + 6-view workload,
-# from rocprofv3 FETCH_SIZE and WRITE_SIZE scaled by a copy of known size
-# (profiles/k_propagate_c1_1920x960_6views.md); None for other workloads.
-TRAFFIC_BYTES_PER_LAUNCH_C1 = int((1_385_000 * 2.00 + 43_430 * 1.00) * 1024)
+# L2-to-memory bytes per k_propagate_q launch on the 1920x960 + 6-view
+# workload, from rocprofv3 FETCH_SIZE and WRITE_SIZE scaled by a copy of known
+# size (profiles/k_propagate_q_c1_1920x960_6views.md); None for other workloads.
+TRAFFIC_BYTES_PER_LAUNCH_C1 = int((1_433_671 * 2.00 + 42_982 * 1.00) * 1024)
 
 
 def ops_per_eval(T: int, N: int) -> int:

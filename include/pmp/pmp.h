@@ -49,6 +49,7 @@ enum { PMP_EDGE_WRAP = 0, PMP_EDGE_CLAMP = 1, PMP_EDGE_ZERO = 2 };
 enum { PMP_COVER_REF_GRID = 0, PMP_COVER_FULL = 1 };
 enum { PMP_INIT_REF_UNIFORM = 0, PMP_INIT_INVDEPTH_HEMI = 1 };
 enum { PMP_MAPS_WORK = 0, PMP_MAPS_INIT = 1, PMP_MAPS_FINAL = 2 };
+enum { PMP_COLOR_GRAY = 0, PMP_COLOR_RGB = 1, PMP_COLOR_BGR = 2 };   /* camera::color_order_t */
 
 typedef struct pmp_ctx pmp_ctx;
 
@@ -112,6 +113,23 @@ int pmp_synchronize(pmp_ctx* ctx);
 int pmp_image_put(pmp_ctx* ctx, uint64_t id, const uint8_t* pixels, int32_t width, int32_t height,
                   int64_t pitch, int32_t on_device);
 int pmp_image_drop(pmp_ctx* ctx, uint64_t id);
+/* Copy a pooled image (width*height bytes, rows packed) to host memory. */
+int pmp_image_get(pmp_ctx* ctx, uint64_t id, uint8_t* pixels);
+
+/* The frame thumbnails of tracking_module.cc:82-93 and data/frame.cc:53-54,
+ * :136-138, on the device: a full-resolution camera image (channels 1, 3 or 4
+ * in color_order) converted to grey (cv::cvtColor) at full resolution, then
+ * resized with INTER_LINEAR (cv::resize) to out_width x out_height and pooled
+ * under id (replacing it).  8-bit fixed point as OpenCV's published formulas.
+ * on_device != 0: pixels is a device pointer. */
+int pmp_image_put_thumbnail(pmp_ctx* ctx, uint64_t id, const uint8_t* pixels, int32_t width, int32_t height,
+                            int64_t pitch, int32_t channels, int32_t color_order, int32_t out_width,
+                            int32_t out_height, int32_t on_device);
+/* cv::resize INTER_LINEAR of an 8-bit image with 1, 3 or 4 interleaved
+ * channels (the colour and mask thumbnails, frame.cc:53, :136-138); src and
+ * dst both host or both device (on_device); dst rows are out_width*channels. */
+int pmp_resize(pmp_ctx* ctx, const uint8_t* src, int32_t width, int32_t height, int64_t pitch, int32_t channels,
+               uint8_t* dst, int32_t out_width, int32_t out_height, int32_t on_device);
 /* Number of images and device bytes held by the pool. */
 int pmp_image_pool_stats(const pmp_ctx* ctx, int64_t* count, int64_t* bytes);
 
@@ -148,6 +166,31 @@ int pmp_set_mask(pmp_ctx* ctx, int64_t mask_id);
  * (depthmap.cc:673-687, :731-767, :1169-1208).  mask: host H*W bytes,
  * 0 = ignored pixel; NULL = the pmp_set_mask image, or all valid. */
 int pmp_finalize(pmp_ctx* ctx, const pmp_params* params, const uint8_t* mask);
+
+/* ---- pre-initialisation (depthmap.cc:690-691), between
+ * pmp_random_initialization and pmp_red_black_pass.  Candidates are scored
+ * with params' scorer at scored pixels; a candidate replaces a pixel's
+ * plane/depth/score (working maps and the copy-back snapshot) when it has the
+ * highest score of the pixel's candidates (the first in loop order among
+ * equal ones) and beats the pixel's score: the sequential outcome of the
+ * reference's OpenMP loops.  mask: host H*W bytes (0 = ignored), NULL = the
+ * pmp_set_mask image or none.  accepted (nullable): pixels changed; asking
+ * for it waits for the device. */
+
+/* PreInitSparsePCL2 (depthmap.cc:361-432): n landmarks (world xyz, doubles)
+ * seed the pixel each reprojects to with the camera-facing plane at its
+ * distance. */
+int pmp_preinit_landmarks(pmp_ctx* ctx, const pmp_params* params, int64_t n, const double* xyz,
+                          const uint8_t* mask, int64_t* accepted);
+/* MemorizeResult (depthmap.cc:808-829): keep the final plane and depth maps
+ * and the reference pose of the last finalize for pmp_preinit_warp. */
+int pmp_memorize_result(pmp_ctx* ctx);
+/* PreInitResultWarp (depthmap.cc:434-516): move every pixel of the memorised
+ * result with depth >= 1e-4 along its plane into the current reference; the
+ * candidate's normal is the memorised plane's direction, as in the reference
+ * (not rotated into the current frame).  No-op before the first
+ * pmp_memorize_result. */
+int pmp_preinit_warp(pmp_ctx* ctx, const pmp_params* params, const uint8_t* mask, int64_t* accepted);
 
 /* The whole estimator: init, params->iterations red-black iterations,
  * finalize.  Replaces the device part of DepthmapEstimator::ComputePatchMatch
@@ -324,6 +367,38 @@ int pmp_pruner_pop_head(pmp_ctx* ctx);           /* PopHead (depthmap.cc:1429-14
 int pmp_pruner_reset(pmp_ctx* ctx);              /* reset (depthmap.cc:1478-1499) */
 int pmp_pruner_size(const pmp_ctx* ctx);         /* queue length */
 int pmp_pruner_get_map(pmp_ctx* ctx, int32_t index, float* depth);  /* read back a queued map */
+
+/* ---- output formats (host only; no context needed) ----------------------- */
+
+/* The equirectangular camera of the map (camera/equirectangular.cc:93-103). */
+typedef struct pmp_camera_info {
+    const char* name;
+    int32_t cols, rows;
+    double fps;
+} pmp_camera_info;
+
+/* One densified keyframe: its pose and the DepthmapPrunerResult of
+ * depthmap.h:243-252 (points3d world xyz, colors BGR, keypts (row, col)). */
+typedef struct pmp_dense_keyframe {
+    uint64_t id;
+    uint64_t src_frm_id;
+    double timestamp;
+    pmp_pose pose;            /* world->camera */
+    int64_t n_points;
+    const double* points;     /* n*3 */
+    const uint8_t* colors;    /* n*3, B G R */
+    const int32_t* keypts;    /* n*2, row col */
+} pmp_dense_keyframe;
+
+/* save_message_pack_dense (io/map_database_io.cc:22-44): MessagePack of
+ * {"cameras", "keyframes", "landmarks"} with one landmark per point
+ * (dense_pointcloud_module.cc:153-174), the file export_dense_msg_to_ply.py
+ * reads. */
+int pmp_write_dense_msgpack(const char* path, const pmp_camera_info* camera, int32_t n_keyframes,
+                            const pmp_dense_keyframe* keyframes);
+/* save_point_cloud_dense (io/map_database_io.cc:46-81): ASCII PLY with
+ * float x y z and uchar red green blue per vertex; rgb: n*3 (R G B). */
+int pmp_write_ply(const char* path, int64_t n, const double* xyz, const uint8_t* rgb);
 
 /* ---- diagnostics --------------------------------------------------------- */
 

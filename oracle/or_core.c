@@ -1028,3 +1028,241 @@ EXPORT int or_params_layout(int64_t* out) {
     out[3] = (int64_t)offsetof(or_params, median_rel_thr);
     return 0;
 }
+
+
+/* ------------------------------------------------------------------ */
+/* host pre-initialisation (DepthmapEstimator::ComputePatchMatch,      */
+/* depthmap.cc:690-691), restated without the OpenMP race              */
+/* ------------------------------------------------------------------ */
+
+/* PlanePosFromDepthAndNormal_equirect (depthmap.cc:130-141): n / (n.(d b)),
+ * the denominator pushed away from 0 to +-1e-5 */
+static inline void plane_from_depth_normal_eps(float bx, float by, float bz, float dep, float nx, float ny,
+                                               float nz, float* p) {
+    float qx = dep * bx, qy = dep * by, qz = dep * bz;
+    float den = fmaf(nz, qz, fmaf(ny, qy, nx * qx));
+    if (den < 1e-5f && den >= 0.0f) den = 1e-5f;
+    else if (den > -1e-5f && den < 0.0f) den = -1e-5f;
+    float r = or_rcp(den);
+    p[0] = nx * r; p[1] = ny * r; p[2] = nz * r;
+}
+
+typedef struct { int pix; float plane[3]; float depth; float score; } cand_t;
+
+/* target pixel of a world point: ReprojectEquirect (depthmap.cc:143-161) then
+ * int(x + 0.5), int(y + 0.5) (:382-383, :465-466); -1 if too near, outside the
+ * image or masked */
+static int target_pixel(const geo_t* g, const double* pose, const double* Xw, const uint8_t* mask, float* dist) {
+    float x, y;
+    if (reproject(g, pose, Xw, &x, &y, dist)) return -1;
+    int j = (int)(x + 0.5f), i = (int)(y + 0.5f);
+    if (i < 0 || j < 0 || i >= g->H || j >= g->W) return -1;
+    if (mask && !mask[(size_t)i * g->W + j]) return -1;
+    return i * g->W + j;
+}
+
+/* Sequential acceptance of the candidates in index order (the loop order of
+ * the reference without its OpenMP race): a candidate replaces the pixel's
+ * plane/depth/score when its score is strictly higher than the pixel's
+ * current score (:400, :490).  Written to the working maps and to the
+ * snapshot kept for the copy-back (the reference's result maps). */
+static int apply_candidates(int W, int H, const cand_t* c, int64_t n, float* plane4, float* depth, float* score,
+                            float* iplane4, float* idepth, float* iscore) {
+    int accepted = 0;   /* pixels changed */
+    uint8_t* changed = (uint8_t*)calloc((size_t)W * H, 1);
+    for (int64_t q = 0; q < n; ++q) {
+        if (c[q].pix < 0) continue;
+        size_t k = (size_t)c[q].pix;
+        if (c[q].score > score[k]) {
+            for (int a = 0; a < 3; ++a) plane4[4 * k + a] = iplane4[4 * k + a] = c[q].plane[a];
+            plane4[4 * k + 3] = iplane4[4 * k + 3] = 0.0f;
+            depth[k] = idepth[k] = c[q].depth;
+            score[k] = iscore[k] = c[q].score;
+            if (!changed[k]) ++accepted;
+            changed[k] = 1;
+        }
+    }
+    free(changed);
+    return accepted;
+}
+
+static void score_candidates(const derived_t* d, const or_params* p, const or_keyframe* kf, cand_t* c, int64_t n) {
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t q = 0; q < n; ++q) {
+        if (c[q].pix < 0) continue;
+        int i = c[q].pix / d->W, j = c[q].pix % d->W;
+        refstat_t rs = ref_stats(d, kf->ref, i, j);
+        c[q].score = score_plane_f(d, p, kf, &rs, i, j, c[q].plane[0], c[q].plane[1], c[q].plane[2], NULL);
+    }
+}
+
+/* PreInitSparsePCL2 (depthmap.cc:361-432): each landmark (world xyz, double)
+ * seeds the pixel it reprojects to with the plane facing the camera (normal
+ * -bearing of that pixel, :393-395) at the landmark's distance (:388), if its
+ * score beats the pixel's.  Candidates are scored with this build's scorer
+ * (params' aggregation and edge mode) and only at scored pixels
+ * (in_score_grid); the reference scores them with its CPU scorer
+ * (ComputePlaneScore, :1104-1127).  mask: nullable.  out_scores (nullable,
+ * n): candidate scores, NaN where no candidate.  Returns the number of pixels
+ * changed. */
+EXPORT int or_preinit_landmarks(const or_keyframe* kf, const or_params* p, float* plane4, float* depth, float* score,
+                                float* iplane4, float* idepth, float* iscore, const uint8_t* mask, int64_t n,
+                                const double* xyz, float* out_scores) {
+    derived_t d;
+    if (derive(kf, p, &d)) return -1;
+    geo_t g;
+    geo_init(&g, d.W, d.H);
+    cand_t* c = (cand_t*)malloc(sizeof(cand_t) * (size_t)(n > 0 ? n : 1));
+    for (int64_t q = 0; q < n; ++q) {
+        float dist;
+        c[q].pix = target_pixel(&g, kf->ref_pose, xyz + 3 * q, mask, &dist);
+        if (c[q].pix >= 0 && !in_score_grid(&d, p, c[q].pix / d.W, c[q].pix % d.W)) c[q].pix = -1;
+        if (c[q].pix < 0) continue;
+        float bx, by, bz;
+        bearing_f(&d, c[q].pix / d.W, c[q].pix % d.W, &bx, &by, &bz);
+        plane_from_depth_normal_eps(bx, by, bz, dist, -bx, -by, -bz, c[q].plane);
+        c[q].depth = dist;
+    }
+    score_candidates(&d, p, kf, c, n);
+    if (out_scores)
+        for (int64_t q = 0; q < n; ++q) out_scores[q] = c[q].pix >= 0 ? c[q].score : or_u2f(NAN_BITS);
+    int acc = apply_candidates(d.W, d.H, c, n, plane4, depth, score, iplane4, idepth, iscore);
+    free(c);
+    geo_free(&g);
+    release(&d);
+    return acc;
+}
+
+/* PosPatchPartOnPlane (depthmap.cc:163-177): the point of plane p on the
+ * bearing of pixel (i, j) of a camera at pose, in the world frame.  scale =
+ * p.b in float (the reference rounds its double dot to float), pushed away
+ * from 0 to +-1e-6; the camera-frame point b/scale is formed in float, the
+ * rigid transform R^T (X - t) in double. */
+static inline void pos_on_plane(const geo_t* g, const double* pose, int i, int j, const float* pl, double* Xw) {
+    float bx = g->rowtab[2 * i] * g->coltab[2 * j];
+    float by = -g->rowtab[2 * i + 1];
+    float bz = g->rowtab[2 * i] * g->coltab[2 * j + 1];
+    float s = fmaf(pl[2], bz, fmaf(pl[1], by, pl[0] * bx));
+    if (s < 1e-6f && s >= 0.0f) s = 1e-6f;
+    else if (s > -1e-6f && s < 0.0f) s = -1e-6f;
+    float r = or_rcp(s);
+    double q0 = (double)(bx * r) - pose[9], q1 = (double)(by * r) - pose[10], q2 = (double)(bz * r) - pose[11];
+    const double* R = pose;
+    Xw[0] = R[0] * q0 + R[3] * q1 + R[6] * q2;
+    Xw[1] = R[1] * q0 + R[4] * q1 + R[7] * q2;
+    Xw[2] = R[2] * q0 + R[5] * q1 + R[8] * q2;
+}
+
+/* PreInitResultWarp (depthmap.cc:434-516): every pixel (r, c) of the
+ * memorised previous result (MemorizeResult, :808-829: the final plane and
+ * depth maps and the pose of the previous reference) with depth >= 1e-4 and
+ * the CURRENT mask set at (r, c) (:445) is moved along its plane into the
+ * current reference camera.  The candidate at the pixel it lands on has the
+ * point's distance as depth and the previous plane's direction, normalised
+ * and not rotated into the current frame (:479-481), as its normal.
+ * Candidates in raster order of (r, c); same scoring and acceptance as
+ * or_preinit_landmarks.  prev maps have the current keyframe's size. */
+EXPORT int or_preinit_warp(const or_keyframe* kf, const or_params* p, float* plane4, float* depth, float* score,
+                           float* iplane4, float* idepth, float* iscore, const uint8_t* mask, const double* prev_pose,
+                           const float* prev_plane4, const float* prev_depth, float* out_scores) {
+    derived_t d;
+    if (derive(kf, p, &d)) return -1;
+    geo_t g;
+    geo_init(&g, d.W, d.H);
+    int64_t n = (int64_t)d.W * d.H;
+    cand_t* c = (cand_t*)malloc(sizeof(cand_t) * (size_t)n);
+#pragma omp parallel for schedule(static)
+    for (int64_t q = 0; q < n; ++q) {
+        int r = (int)(q / d.W), cc = (int)(q % d.W);
+        c[q].pix = -1;
+        if (mask && !mask[q]) continue;
+        if (!(prev_depth[q] >= 1e-4f)) continue;
+        const float* pp = prev_plane4 + 4 * q;
+        double Xw[3];
+        pos_on_plane(&g, prev_pose, r, cc, pp, Xw);
+        float dist;
+        int pix = target_pixel(&g, kf->ref_pose, Xw, mask, &dist);
+        if (pix < 0 || !in_score_grid(&d, p, pix / d.W, pix % d.W)) continue;
+        float nr2 = or_clamp(fmaf(pp[2], pp[2], fmaf(pp[1], pp[1], pp[0] * pp[0])), 1e-30f, 1e30f);
+        float inr = or_rsqrt_pos(nr2);
+        float bx, by, bz;
+        bearing_f(&d, pix / d.W, pix % d.W, &bx, &by, &bz);
+        plane_from_depth_normal_eps(bx, by, bz, dist, pp[0] * inr, pp[1] * inr, pp[2] * inr, c[q].plane);
+        c[q].depth = dist;
+        c[q].pix = pix;
+    }
+    score_candidates(&d, p, kf, c, n);
+    if (out_scores)
+        for (int64_t q = 0; q < n; ++q) out_scores[q] = c[q].pix >= 0 ? c[q].score : or_u2f(NAN_BITS);
+    int acc = apply_candidates(d.W, d.H, c, n, plane4, depth, score, iplane4, idepth, iscore);
+    free(c);
+    geo_free(&g);
+    release(&d);
+    return acc;
+}
+
+/* ------------------------------------------------------------------ */
+/* thumbnails: the frame's grey / mask / colour thumbnails             */
+/* (tracking_module.cc:82-93, data/frame.cc:53-54, :136-138), i.e.     */
+/* cv::cvtColor(BGR2GRAY) then cv::resize(INTER_LINEAR) on 8-bit data. */
+/* OpenCV 4.1.0 is absent here; both follow its published 8-bit        */
+/* fixed-point formulas:                                               */
+/*   grey  Y = (1868 B + 9617 G + 4899 R + 2^13) >> 14                 */
+/*   resize: source coordinate f = (d + 0.5) * (src/dst) - 0.5 (float), */
+/*   s = floor(f), f -= s, clamped to the first / last sample (f = 0);  */
+/*   weights round(1 - f) * 2048 and round(f * 2048) (shorts, rounded  */
+/*   independently); rows then columns in int, (v + 2^21) >> 22.       */
+/* ------------------------------------------------------------------ */
+
+/* color order: 0 grey, 1 RGB, 2 BGR (camera::color_order_t) */
+EXPORT int or_to_gray(const uint8_t* src, int W, int H, int64_t pitch, int channels, int order, uint8_t* dst) {
+    if (channels != 1 && channels != 3 && channels != 4) return -1;
+    for (int i = 0; i < H; ++i)
+        for (int j = 0; j < W; ++j) {
+            const uint8_t* p = src + (size_t)i * (size_t)pitch + (size_t)j * channels;
+            if (channels == 1 || order == 0) { dst[(size_t)i * W + j] = p[0]; continue; }
+            int r = order == 1 ? p[0] : p[2], g = p[1], b = order == 1 ? p[2] : p[0];
+            dst[(size_t)i * W + j] = (uint8_t)((b * 1868 + g * 9617 + r * 4899 + (1 << 13)) >> 14);
+        }
+    return 0;
+}
+
+/* per destination index: source index s0, s1 and weights a0, a1 (sum ~ 2048) */
+EXPORT void or_resize_taps(int src, int dst, int32_t* s0, int32_t* s1, int16_t* a0, int16_t* a1) {
+    const double scale = (double)src / (double)dst;
+    for (int d = 0; d < dst; ++d) {
+        float f = (float)((d + 0.5) * scale - 0.5);
+        int s = (int)floorf(f);
+        f -= (float)s;
+        if (s < 0) { f = 0.0f; s = 0; }
+        if (s >= src - 1) { f = 0.0f; s = src - 1; }
+        s0[d] = s;
+        s1[d] = s + 1 < src ? s + 1 : src - 1;
+        a0[d] = (int16_t)lrintf((1.0f - f) * 2048.0f);
+        a1[d] = (int16_t)lrintf(f * 2048.0f);
+    }
+}
+
+EXPORT int or_resize_linear(const uint8_t* src, int W, int H, int64_t pitch, int channels, uint8_t* dst, int dW,
+                            int dH) {
+    if (W < 1 || H < 1 || dW < 1 || dH < 1 || channels < 1 || channels > 4) return -1;
+    int32_t* xs0 = (int32_t*)malloc(sizeof(int32_t) * dW); int32_t* xs1 = (int32_t*)malloc(sizeof(int32_t) * dW);
+    int16_t* xa0 = (int16_t*)malloc(sizeof(int16_t) * dW); int16_t* xa1 = (int16_t*)malloc(sizeof(int16_t) * dW);
+    int32_t* ys0 = (int32_t*)malloc(sizeof(int32_t) * dH); int32_t* ys1 = (int32_t*)malloc(sizeof(int32_t) * dH);
+    int16_t* ya0 = (int16_t*)malloc(sizeof(int16_t) * dH); int16_t* ya1 = (int16_t*)malloc(sizeof(int16_t) * dH);
+    or_resize_taps(W, dW, xs0, xs1, xa0, xa1);
+    or_resize_taps(H, dH, ys0, ys1, ya0, ya1);
+    for (int y = 0; y < dH; ++y) {
+        const uint8_t* r0 = src + (size_t)ys0[y] * (size_t)pitch;
+        const uint8_t* r1 = src + (size_t)ys1[y] * (size_t)pitch;
+        for (int x = 0; x < dW; ++x)
+            for (int c = 0; c < channels; ++c) {
+                int h0 = r0[xs0[x] * channels + c] * xa0[x] + r0[xs1[x] * channels + c] * xa1[x];
+                int h1 = r1[xs0[x] * channels + c] * xa0[x] + r1[xs1[x] * channels + c] * xa1[x];
+                int v = (h0 * ya0[y] + h1 * ya1[y] + (1 << 21)) >> 22;
+                dst[((size_t)y * dW + x) * channels + c] = (uint8_t)(v < 0 ? 0 : (v > 255 ? 255 : v));
+            }
+    }
+    free(xs0); free(xs1); free(xa0); free(xa1); free(ys0); free(ys1); free(ya0); free(ya1);
+    return 0;
+}

@@ -64,6 +64,13 @@ def lib():
         L.or_params_layout.argtypes = [_i64p]
         L.or_params_layout.restype = ctypes.c_int
         L.or_derive_poses.argtypes = [_f64p, ctypes.c_int, _f64p, _f32p, _f32p]
+        L.or_to_gray.argtypes = [_u8p, ctypes.c_int, ctypes.c_int, ctypes.c_int64, ctypes.c_int, ctypes.c_int, _u8p]
+        L.or_resize_linear.argtypes = [_u8p, ctypes.c_int, ctypes.c_int, ctypes.c_int64, ctypes.c_int, _u8p,
+                                       ctypes.c_int, ctypes.c_int]
+        L.or_preinit_landmarks.argtypes = [P(OrKeyframe), P(Params), _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, _u8p,
+                                           ctypes.c_int64, _f64p, _f32p]
+        L.or_preinit_warp.argtypes = [P(OrKeyframe), P(Params), _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, _u8p,
+                                      _f64p, _f32p, _f32p, _f32p]
         for name in ("or_init", "or_score_field", "or_score_field_f64", "or_propagate", "or_propagate_rows",
                      "or_finalize", "or_densify", "or_clean", "or_prune", "or_eval_math", "or_refine_normals",
                      "or_tables", "or_derive_poses"):
@@ -73,7 +80,7 @@ def lib():
 
 
 def _p(a, t):
-    return a.ctypes.data_as(t)
+    return None if a is None else a.ctypes.data_as(t)
 
 
 def set_threads(n: int):
@@ -266,3 +273,55 @@ def params_layout():
     out = np.zeros(4, np.int64)
     lib().or_params_layout(_p(out, _i64p))
     return [int(x) for x in out]
+
+
+def _maps6(work, init):
+    out = [np.array(a, dtype=np.float32, copy=True) for a in (*work, *init)]
+    return out
+
+
+def preinit_landmarks(kf: Keyframe, p: Params, work, init, mask, xyz):
+    """PreInitSparsePCL2 restated.  work/init: (plane4, depth, score) each.
+    Returns (work', init', candidate scores, accepted count)."""
+    m = _maps6(work, init)
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+    sc = np.zeros(len(xyz), np.float32)
+    mk = None if mask is None else np.ascontiguousarray(mask, dtype=np.uint8)
+    n = lib().or_preinit_landmarks(ctypes.byref(kf.c), ctypes.byref(p), *[_p(a, _f32p) for a in m], _p(mk, _u8p),
+                                   len(xyz), _p(xyz, _f64p), _p(sc, _f32p))
+    _chk(min(n, 0), "preinit_landmarks")
+    return tuple(m[:3]), tuple(m[3:]), sc, n
+
+
+def preinit_warp(kf: Keyframe, p: Params, work, init, mask, prev_pose, prev_plane4, prev_depth):
+    """PreInitResultWarp restated.  Returns (work', init', candidate scores, accepted count)."""
+    m = _maps6(work, init)
+    pp = np.ascontiguousarray(prev_pose, dtype=np.float64).reshape(12)
+    pl = np.ascontiguousarray(prev_plane4, dtype=np.float32)
+    pd = np.ascontiguousarray(prev_depth, dtype=np.float32)
+    sc = np.zeros(pd.size, np.float32)
+    mk = None if mask is None else np.ascontiguousarray(mask, dtype=np.uint8)
+    n = lib().or_preinit_warp(ctypes.byref(kf.c), ctypes.byref(p), *[_p(a, _f32p) for a in m], _p(mk, _u8p),
+                              _p(pp, _f64p), _p(pl, _f32p), _p(pd, _f32p), _p(sc, _f32p))
+    _chk(min(n, 0), "preinit_warp")
+    return tuple(m[:3]), tuple(m[3:]), sc.reshape(pd.shape), n
+
+
+def to_gray(img, order=2):
+    """cvtColor to grey of an (H, W) or (H, W, C) uint8 image; order 0 grey, 1 RGB, 2 BGR."""
+    img = np.ascontiguousarray(img, dtype=np.uint8)
+    H, W = img.shape[:2]
+    C = 1 if img.ndim == 2 else img.shape[2]
+    out = np.zeros((H, W), np.uint8)
+    _chk(lib().or_to_gray(_p(img, _u8p), W, H, W * C, C, order, _p(out, _u8p)), "to_gray")
+    return out
+
+
+def resize_linear(img, dW, dH):
+    """cv::resize INTER_LINEAR of an 8-bit image with 1..4 channels."""
+    img = np.ascontiguousarray(img, dtype=np.uint8)
+    H, W = img.shape[:2]
+    C = 1 if img.ndim == 2 else img.shape[2]
+    out = np.zeros((dH, dW) if C == 1 and img.ndim == 2 else (dH, dW, C), np.uint8)
+    _chk(lib().or_resize_linear(_p(img, _u8p), W, H, W * C, C, _p(out, _u8p), dW, dH), "resize_linear")
+    return out

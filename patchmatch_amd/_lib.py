@@ -48,6 +48,8 @@ EXPORTS = [
     "pmp_batch_densify", "pmp_batch_get_maps", "pmp_batch_set_config", "pmp_batch_get_config",
     "pmp_batch_autotune", "pmp_server_create", "pmp_server_destroy", "pmp_server_last_error",
     "pmp_server_submit", "pmp_server_wait", "pmp_server_fetch", "pmp_server_stats", "pmp_server_pause",
+    "pmp_preinit_landmarks", "pmp_memorize_result", "pmp_preinit_warp",
+    "pmp_write_dense_msgpack", "pmp_write_ply", "pmp_image_put_thumbnail", "pmp_resize", "pmp_image_get",
 ]
 
 PMP_MAX_VIEWS = 10
@@ -108,6 +110,17 @@ class ServerInfo(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int64) for n in ("submitted", "completed", "failed", "steps", "pool_images",
                                               "pool_bytes", "maps_bytes", "uploads", "evictions")] + \
                [(n, ctypes.c_int32) for n in ("in_flight", "queued", "max_in_flight", "reserved")]
+
+
+class CameraInfo(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char_p), ("cols", ctypes.c_int32), ("rows", ctypes.c_int32),
+                ("fps", ctypes.c_double)]
+
+
+class DenseKeyframe(ctypes.Structure):
+    _fields_ = [("id", ctypes.c_uint64), ("src_frm_id", ctypes.c_uint64), ("timestamp", ctypes.c_double),
+                ("pose", Pose), ("n_points", ctypes.c_int64), ("points", _f64p), ("colors", _u8p),
+                ("keypts", _i32p)]
 
 
 _lib = None
@@ -171,6 +184,14 @@ def load():
         "pmp_server_fetch": ([_vp, u64, _f32p, _f32p, _f32p, _i32p, P(RequestInfo)], c_int),
         "pmp_server_stats": ([_vp, P(ServerInfo)], c_int),
         "pmp_server_pause": ([_vp, i32], c_int),
+        "pmp_preinit_landmarks": ([_vp, P(Params), i64, _f64p, _u8p, _i64p], c_int),
+        "pmp_memorize_result": ([_vp], c_int),
+        "pmp_preinit_warp": ([_vp, P(Params), _u8p, _i64p], c_int),
+        "pmp_write_dense_msgpack": ([ctypes.c_char_p, P(CameraInfo), i32, P(DenseKeyframe)], c_int),
+        "pmp_write_ply": ([ctypes.c_char_p, i64, _f64p, _u8p], c_int),
+        "pmp_image_put_thumbnail": ([_vp, u64, _u8p, i32, i32, i64, i32, i32, i32, i32, i32], c_int),
+        "pmp_resize": ([_vp, _u8p, i32, i32, i64, i32, _u8p, i32, i32, i32], c_int),
+        "pmp_image_get": ([_vp, u64, _u8p], c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -182,6 +203,35 @@ def load():
 
 def _p(a, t):
     return None if a is None else a.ctypes.data_as(t)
+
+
+def write_ply(path: str, xyz, rgb):
+    """ASCII PLY of n points (world xyz doubles, R G B bytes); no GPU needed."""
+    L = load()
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+    rgb = np.ascontiguousarray(rgb, dtype=np.uint8).reshape(-1, 3)
+    rc = L.pmp_write_ply(path.encode(), len(xyz), _p(xyz, _f64p), _p(rgb, _u8p))
+    if rc != 0:
+        raise PmpError(f"pmp_write_ply: {STATUS.get(rc, rc)}")
+
+
+def write_dense_msgpack(path: str, camera: str, cols: int, rows: int, keyframes, fps: float = 30.0):
+    """Dense map as MessagePack.  keyframes: dicts with id, pose (12), points (n,3),
+    colors_bgr (n,3), keypts (n,2 row col), optional src_frm_id and timestamp."""
+    L = load()
+    keep = []
+    arr = (DenseKeyframe * max(len(keyframes), 1))()
+    for k, kf in enumerate(keyframes):
+        pts = np.ascontiguousarray(kf["points"], dtype=np.float64).reshape(-1, 3)
+        col = np.ascontiguousarray(kf["colors_bgr"], dtype=np.uint8).reshape(-1, 3)
+        kps = np.ascontiguousarray(kf["keypts"], dtype=np.int32).reshape(-1, 2)
+        keep += [pts, col, kps]
+        arr[k] = DenseKeyframe(kf["id"], kf.get("src_frm_id", kf["id"]), kf.get("timestamp", 0.0),
+                               Pose.from12(kf["pose"]), len(pts), _p(pts, _f64p), _p(col, _u8p), _p(kps, _i32p))
+    cam = CameraInfo(camera.encode(), cols, rows, fps)
+    rc = L.pmp_write_dense_msgpack(path.encode(), ctypes.byref(cam), len(keyframes), arr)
+    if rc != 0:
+        raise PmpError(f"pmp_write_dense_msgpack: {STATUS.get(rc, rc)}")
 
 
 class Context:
@@ -227,6 +277,30 @@ class Context:
     def image_put_device(self, image_id: int, ptr: int, W: int, H: int, pitch: int):
         self._chk(self.L.pmp_image_put(self.h, image_id, ctypes.cast(ptr, _u8p), W, H, pitch, 1), "pmp_image_put")
 
+    def image_put_thumbnail(self, image_id: int, img, out_shape, color_order=2):
+        """Grey thumbnail (cvtColor + INTER_LINEAR resize) of an (H, W[, C]) uint8 image, pooled under image_id."""
+        img = np.ascontiguousarray(img, dtype=np.uint8)
+        H, W = img.shape[:2]
+        C = 1 if img.ndim == 2 else img.shape[2]
+        oH, oW = out_shape
+        self._chk(self.L.pmp_image_put_thumbnail(self.h, image_id, _p(img, _u8p), W, H, W * C, C, color_order, oW,
+                                                 oH, 0), "pmp_image_put_thumbnail")
+
+    def resize(self, img, out_shape):
+        img = np.ascontiguousarray(img, dtype=np.uint8)
+        H, W = img.shape[:2]
+        C = 1 if img.ndim == 2 else img.shape[2]
+        oH, oW = out_shape
+        out = np.zeros((oH, oW) if img.ndim == 2 else (oH, oW, C), np.uint8)
+        self._chk(self.L.pmp_resize(self.h, _p(img, _u8p), W, H, W * C, C, _p(out, _u8p), oW, oH, 0), "pmp_resize")
+        return out
+
+    def image_get(self, image_id: int, shape):
+        """Copy a pooled grey image back to the host."""
+        out = np.zeros(shape, np.uint8)
+        self._chk(self.L.pmp_image_get(self.h, image_id, _p(out, _u8p)), "pmp_image_get")
+        return out
+
     def image_drop(self, image_id: int):
         self._chk(self.L.pmp_image_drop(self.h, image_id), "pmp_image_drop")
 
@@ -267,6 +341,24 @@ class Context:
     def densify(self, p: Params, mask=None):
         m = None if mask is None else np.ascontiguousarray(mask, dtype=np.uint8)
         self._chk(self.L.pmp_densify(self.h, ctypes.byref(p), _p(m, _u8p)), "pmp_densify")
+
+    def preinit_landmarks(self, p: Params, xyz, mask=None) -> int:
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+        m = None if mask is None else np.ascontiguousarray(mask, dtype=np.uint8)
+        acc = ctypes.c_int64()
+        self._chk(self.L.pmp_preinit_landmarks(self.h, ctypes.byref(p), len(xyz), _p(xyz, _f64p), _p(m, _u8p),
+                                               ctypes.byref(acc)), "pmp_preinit_landmarks")
+        return acc.value
+
+    def memorize_result(self):
+        self._chk(self.L.pmp_memorize_result(self.h), "pmp_memorize_result")
+
+    def preinit_warp(self, p: Params, mask=None) -> int:
+        m = None if mask is None else np.ascontiguousarray(mask, dtype=np.uint8)
+        acc = ctypes.c_int64()
+        self._chk(self.L.pmp_preinit_warp(self.h, ctypes.byref(p), _p(m, _u8p), ctypes.byref(acc)),
+                  "pmp_preinit_warp")
+        return acc.value
 
     def set_mask(self, mask_id: int):
         self._chk(self.L.pmp_set_mask(self.h, mask_id), "pmp_set_mask")

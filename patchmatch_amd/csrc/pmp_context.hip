@@ -242,9 +242,10 @@ void pmp_destroy(pmp_ctx* ctx) {
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     queue_free(ctx);
+    preinit_free(ctx);
     free_slot(ctx->ws);
     for (auto& kv : ctx->images) free_image(kv.second);
-    void* bufs[] = {ctx->mask, ctx->counters,
+    void* bufs[] = {ctx->mask, ctx->counters, ctx->thumb.src, ctx->thumb.gray, ctx->thumb.dst, ctx->thumb.taps,
                     ctx->clean_maps, ctx->clean_poses, ctx->prune_z, ctx->prune_alive, ctx->prune_out,
                     ctx->prune_pts, ctx->prune_cols, ctx->prune_kps, ctx->prune_blk, ctx->prune_poses, ctx->dbg};
     for (void* b : bufs)
@@ -353,6 +354,15 @@ int pmp_image_put(pmp_ctx* ctx, uint64_t id, const uint8_t* pixels, int32_t widt
         return check_hip(ctx, e, "pmp_image_put copy");
     }
     ctx->images[id] = im;
+    return PMP_OK;
+}
+
+int pmp_image_get(pmp_ctx* ctx, uint64_t id, uint8_t* pixels) {
+    if (!ctx || !pixels) return PMP_EINVAL;
+    auto it = ctx->images.find(id);
+    if (it == ctx->images.end()) return set_error(ctx, PMP_ENOTFOUND, "unknown image id");
+    PMP_HIP(ctx, hipMemcpyAsync(pixels, it->second.gray, it->second.bytes, hipMemcpyDeviceToHost, ctx->stream));
+    PMP_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return PMP_OK;
 }
 

@@ -107,7 +107,7 @@ int build_args(pmp_ctx* ctx, const pmp_params* p, uint64_t ref_id, int N, const 
 // The current keyframe as a one-slot descriptor set over the workspace, uploaded.
 // dmask (device, nullable) is the finalize mask; with_mask: use the pooled
 // mask of pmp_set_mask when dmask is null.
-static int plain_prepare(pmp_ctx* ctx, const pmp_params* p, const uint8_t* dmask, bool with_mask, int pass0,
+int plain_prepare(pmp_ctx* ctx, const pmp_params* p, const uint8_t* dmask, bool with_mask, int pass0,
                          QueueState*& q, QueueRun& r) {
     if (!ctx->views_set) return set_error(ctx, PMP_ESTATE, "pmp_set_views has not been called");
     int rc = queue_state(ctx, q);
@@ -140,7 +140,7 @@ static int plain_prepare(pmp_ctx* ctx, const pmp_params* p, const uint8_t* dmask
 }
 
 // A host mask goes to the workspace's staging buffer first.
-static int stage_mask(pmp_ctx* ctx, const uint8_t* mask_host, const uint8_t*& dmask) {
+int stage_mask(pmp_ctx* ctx, const uint8_t* mask_host, const uint8_t*& dmask) {
     dmask = nullptr;
     if (!mask_host) return PMP_OK;
     const size_t P = (size_t)ctx->W * ctx->H;

@@ -66,6 +66,7 @@ struct Image {
 };
 
 struct QueueState;   // pmp_queue.h
+struct PreState;     // pmp_preinit.hip
 
 struct PrunerView {
     int W = 0, H = 0;
@@ -129,6 +130,20 @@ struct pmp_ctx {
 
     // descriptors, launch configuration and graphs of the work-queue kernels
     pmp::QueueState* queue = nullptr;
+    // pre-initialisation candidates and the memorised previous result
+    pmp::PreState* pre = nullptr;
+
+    // thumbnail staging (pmp_thumb.hip)
+    struct {
+        uint8_t* src = nullptr;
+        size_t src_cap = 0;
+        uint8_t* gray = nullptr;
+        size_t gray_cap = 0;
+        uint8_t* dst = nullptr;
+        size_t dst_cap = 0;
+        uint8_t* taps = nullptr;
+        size_t taps_cap = 0;
+    } thumb;
 
     // debug
     float* dbg = nullptr;
@@ -159,6 +174,7 @@ void host_tables(int W, int H, std::vector<float2>& rowtab, std::vector<float2>&
 void derive_poses(const double* ref_pose, int N, const double* view_poses, float (*M)[9], float (*c)[3]);
 int check_params(pmp_ctx* ctx, const pmp_params* p);
 void queue_free(pmp_ctx* ctx);
+void preinit_free(pmp_ctx* ctx);
 int ensure_slot(pmp_ctx* ctx, SlotBuf& b, int W, int H);
 void free_slot(SlotBuf& b);
 int build_args(pmp_ctx* ctx, const pmp_params* p, uint64_t ref_id, int N, const uint64_t* view_ids,

@@ -109,5 +109,15 @@ int q_finalize(pmp_ctx* ctx, hipStream_t s, DescSet& d, const QueueRun& r);
 // clear + init + `passes` passes + finalize, replayed from a graph cached in d
 // when use_graph (the set's contents may change between replays).
 int q_densify(pmp_ctx* ctx, hipStream_t s, DescSet& d, const QueueRun& r, int passes, bool use_graph);
+// n words of p set to v with a fill kernel (capture-safe).
+int q_fill_u32(pmp_ctx* ctx, hipStream_t s, unsigned int* p, size_t n, unsigned int v);
+
+// The one-keyframe API's current keyframe as a one-slot set over the context
+// workspace, uploaded (pmp_estimator.hip).  dmask (device, nullable) is the
+// mask; with_mask: use the pmp_set_mask image when dmask is null.
+int plain_prepare(pmp_ctx* ctx, const pmp_params* p, const uint8_t* dmask, bool with_mask, int pass0,
+                  QueueState*& q, QueueRun& r);
+// A host mask (nullable) copied to the workspace's staging buffer.
+int stage_mask(pmp_ctx* ctx, const uint8_t* mask_host, const uint8_t*& dmask);
 
 }  // namespace pmp

@@ -385,6 +385,15 @@ int q_finalize(pmp_ctx* ctx, hipStream_t s, DescSet& d, const QueueRun& r) {
     return PMP_OK;
 }
 
+int q_fill_u32(pmp_ctx* ctx, hipStream_t s, unsigned int* p, size_t n, unsigned int v) {
+    for (size_t off = 0; off < n; off += (size_t)1 << 30) {
+        const size_t m = (n - off) < ((size_t)1 << 30) ? (n - off) : ((size_t)1 << 30);
+        hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, p + off, (int)m, v);
+    }
+    PMP_HIP(ctx, hipGetLastError());
+    return PMP_OK;
+}
+
 static int q_sequence(pmp_ctx* ctx, hipStream_t s, DescSet& d, const QueueRun& r, int passes) {
     int rc;
     if ((rc = q_clear(ctx, s, d, passes))) return rc;

@@ -36,6 +36,7 @@ def main(argv=None):
         ctx.load_keyframe(kf.ref, kf.views, kf.ref_pose, kf.view_poses)
         ctx.image_put(1000, kf.mask)
         ctx.set_mask(1000)
+        ctx.set_timing(True)   # direct launches (no hipGraph): one dispatch per pass for the profiler
         for s in range(args.steps):
             t = time.perf_counter()
             ctx.densify(p)
