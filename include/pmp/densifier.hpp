@@ -152,9 +152,18 @@ public:
         pmp_params p = params_;
         p.min_depth = (float)ref.mind;
         p.max_depth = (float)ref.maxd;
-        check(ctx_.get(), pmp_set_views(ctx_.get(), ref.id, &ref.pose, (int32_t)ids.size(), ids.data(), poses.data()),
-              "pmp_set_views");
-        check(ctx_.get(), pmp_densify(ctx_.get(), &p, ref.mask.data()), "pmp_densify");
+        pmp_ctx* c = ctx_.get();
+        check(c, pmp_set_views(c, ref.id, &ref.pose, (int32_t)ids.size(), ids.data(), poses.data()), "pmp_set_views");
+        // RandomInitialization, then the pre-initialisation (depthmap.cc:656-691):
+        // the reference's landmarks, then the memorised previous result
+        check(c, pmp_random_initialization(c, &p), "pmp_random_initialization");
+        if (!ref.landmarks.empty())
+            check(c, pmp_preinit_landmarks(c, &p, (int64_t)(ref.landmarks.size() / 3), ref.landmarks.data(),
+                                           ref.mask.data(), nullptr),
+                  "pmp_preinit_landmarks");
+        check(c, pmp_preinit_warp(c, &p, ref.mask.data(), nullptr), "pmp_preinit_warp");
+        check(c, pmp_red_black_pass(c, &p, 0, p.iterations), "pmp_red_black_pass");
+        check(c, pmp_finalize(c, &p, ref.mask.data()), "pmp_finalize");
         const size_t P = (size_t)ref.width * ref.height;
         std::vector<float> plane4(4 * P);
         result->rows = ref.height;
@@ -171,6 +180,7 @@ public:
             for (int c = 0; c < 3; ++c) result->plane[3 * k + c] = (double)plane4[4 * k + c];
         std::memcpy(result->R, ref.pose.R, sizeof(result->R));
         std::memcpy(result->t, ref.pose.t, sizeof(result->t));
+        check(c, pmp_memorize_result(c), "pmp_memorize_result");   // MemorizeResult (:768)
     }
 
     // PopHeadUnlessOne (depthmap.cc:241-254)
@@ -182,9 +192,11 @@ public:
         return (int)views_.size();
     }
 
+    // reset (depthmap.cc:256-280): the memorised result is cleared too
     void reset() {
         for (auto& v : views_) pmp_image_drop(ctx_.get(), v.id);
         views_.clear();
+        pmp_forget_result(ctx_.get());
     }
 
 private:

@@ -185,6 +185,9 @@ int pmp_preinit_landmarks(pmp_ctx* ctx, const pmp_params* params, int64_t n, con
 /* MemorizeResult (depthmap.cc:808-829): keep the final plane and depth maps
  * and the reference pose of the last finalize for pmp_preinit_warp. */
 int pmp_memorize_result(pmp_ctx* ctx);
+/* Drop the memorised result (DepthmapEstimator::reset, depthmap.cc:261-270):
+ * pmp_preinit_warp is a no-op until the next pmp_memorize_result. */
+int pmp_forget_result(pmp_ctx* ctx);
 /* PreInitResultWarp (depthmap.cc:434-516): move every pixel of the memorised
  * result with depth >= 1e-4 along its plane into the current reference; the
  * candidate's normal is the memorised plane's direction, as in the reference
@@ -367,6 +370,20 @@ int pmp_pruner_pop_head(pmp_ctx* ctx);           /* PopHead (depthmap.cc:1429-14
 int pmp_pruner_reset(pmp_ctx* ctx);              /* reset (depthmap.cc:1478-1499) */
 int pmp_pruner_size(const pmp_ctx* ctx);         /* queue length */
 int pmp_pruner_get_map(pmp_ctx* ctx, int32_t index, float* depth);  /* read back a queued map */
+
+/* ---- densify-thread front-end (host only; include/pmp/frontend.hpp) ------ */
+
+/* score_view (depthmap_computation_module.cc:147-198): over the landmarks
+ * common to two keyframes (matched by id; positions from cur), the fraction
+ * seen from the two camera centres under an angle in (pi/60, pi/6).  NaN
+ * without common landmarks (the reference's 0/0), which fails the 0.25
+ * gate.  Poses: 12 doubles, world->camera R (row-major) then t. */
+double pmp_score_view(const double* prev_pose, int64_t n_prev, const uint64_t* prev_ids, const double* prev_xyz,
+                      const double* cur_pose, int64_t n_cur, const uint64_t* cur_ids, const double* cur_xyz);
+/* compute_depth_intervall (:123-131) with computePercentileDepth_10_90
+ * (data/keyframe.cc:428-451): min 1e-5, max 1.1 x the landmark distance at
+ * sorted index n/10*9.  PMP_EINVAL for n = 0. */
+int pmp_depth_interval(const double* pose, int64_t n, const double* xyz, double* min_depth, double* max_depth);
 
 /* ---- output formats (host only; no context needed) ----------------------- */
 

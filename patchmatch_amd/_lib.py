@@ -48,8 +48,9 @@ EXPORTS = [
     "pmp_batch_densify", "pmp_batch_get_maps", "pmp_batch_set_config", "pmp_batch_get_config",
     "pmp_batch_autotune", "pmp_server_create", "pmp_server_destroy", "pmp_server_last_error",
     "pmp_server_submit", "pmp_server_wait", "pmp_server_fetch", "pmp_server_stats", "pmp_server_pause",
-    "pmp_preinit_landmarks", "pmp_memorize_result", "pmp_preinit_warp",
+    "pmp_preinit_landmarks", "pmp_memorize_result", "pmp_preinit_warp", "pmp_forget_result",
     "pmp_write_dense_msgpack", "pmp_write_ply", "pmp_image_put_thumbnail", "pmp_resize", "pmp_image_get",
+    "pmp_score_view", "pmp_depth_interval",
 ]
 
 PMP_MAX_VIEWS = 10
@@ -186,12 +187,15 @@ def load():
         "pmp_server_pause": ([_vp, i32], c_int),
         "pmp_preinit_landmarks": ([_vp, P(Params), i64, _f64p, _u8p, _i64p], c_int),
         "pmp_memorize_result": ([_vp], c_int),
+        "pmp_forget_result": ([_vp], c_int),
         "pmp_preinit_warp": ([_vp, P(Params), _u8p, _i64p], c_int),
         "pmp_write_dense_msgpack": ([ctypes.c_char_p, P(CameraInfo), i32, P(DenseKeyframe)], c_int),
         "pmp_write_ply": ([ctypes.c_char_p, i64, _f64p, _u8p], c_int),
         "pmp_image_put_thumbnail": ([_vp, u64, _u8p, i32, i32, i64, i32, i32, i32, i32, i32], c_int),
         "pmp_resize": ([_vp, _u8p, i32, i32, i64, i32, _u8p, i32, i32, i32], c_int),
         "pmp_image_get": ([_vp, u64, _u8p], c_int),
+        "pmp_score_view": ([_f64p, i64, _u64p, _f64p, _f64p, i64, _u64p, _f64p], ctypes.c_double),
+        "pmp_depth_interval": ([_f64p, i64, _f64p, _f64p, _f64p], c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -203,6 +207,28 @@ def load():
 
 def _p(a, t):
     return None if a is None else a.ctypes.data_as(t)
+
+
+def score_view(prev_pose, prev_ids, prev_xyz, cur_pose, cur_ids, cur_xyz) -> float:
+    """Parallax score of the densify thread's keyframe gate (host code)."""
+    L = load()
+    a = [np.ascontiguousarray(x, dtype=np.float64).reshape(-1) for x in (prev_pose, prev_xyz, cur_pose, cur_xyz)]
+    pi = np.ascontiguousarray(prev_ids, dtype=np.uint64)
+    ci = np.ascontiguousarray(cur_ids, dtype=np.uint64)
+    return L.pmp_score_view(_p(a[0], _f64p), len(pi), _p(pi, _u64p), _p(a[1], _f64p), _p(a[2], _f64p), len(ci),
+                            _p(ci, _u64p), _p(a[3], _f64p))
+
+
+def depth_interval(pose, xyz):
+    """(min_depth, max_depth) of a keyframe from its landmarks (host code)."""
+    L = load()
+    pose = np.ascontiguousarray(pose, dtype=np.float64).reshape(12)
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+    lo, hi = ctypes.c_double(), ctypes.c_double()
+    rc = L.pmp_depth_interval(_p(pose, _f64p), len(xyz), _p(xyz, _f64p), ctypes.byref(lo), ctypes.byref(hi))
+    if rc != 0:
+        raise PmpError(f"pmp_depth_interval: {STATUS.get(rc, rc)}")
+    return lo.value, hi.value
 
 
 def write_ply(path: str, xyz, rgb):
@@ -352,6 +378,9 @@ class Context:
 
     def memorize_result(self):
         self._chk(self.L.pmp_memorize_result(self.h), "pmp_memorize_result")
+
+    def forget_result(self):
+        self._chk(self.L.pmp_forget_result(self.h), "pmp_forget_result")
 
     def preinit_warp(self, p: Params, mask=None) -> int:
         m = None if mask is None else np.ascontiguousarray(mask, dtype=np.uint8)

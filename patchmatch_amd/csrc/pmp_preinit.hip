@@ -352,6 +352,12 @@ int pmp_memorize_result(pmp_ctx* ctx) {
     return PMP_OK;
 }
 
+int pmp_forget_result(pmp_ctx* ctx) {
+    if (!ctx) return PMP_EINVAL;
+    if (ctx->pre) ctx->pre->prev_valid = false;
+    return PMP_OK;
+}
+
 int pmp_preinit_warp(pmp_ctx* ctx, const pmp_params* params, const uint8_t* mask, int64_t* accepted) {
     if (!ctx || !params) return PMP_EINVAL;
     if (accepted) *accepted = 0;

@@ -1,11 +1,13 @@
 // facade_window.cpp -- drives pmp::DepthmapEstimator (include/pmp/densifier.hpp)
 // the way the OpenVSLAM densify thread drives dense::DepthmapEstimator
 // (depthmap_computation_module.cc:374-397): AddView per keyframe, and once the
-// window is full, ComputePatchMatch + PopHeadUnlessOne.
+// window is full, ComputePatchMatch (with the landmark and previous-result
+// pre-initialisation) + PopHeadUnlessOne.
 //
 //   facade_window <input.bin> <output.bin>
 // input : int32 W, H, n_views; pmp_params (raw struct); per view: 12 doubles
-//         (R row-major, t), W*H grey bytes, W*H mask bytes, double mind, maxd
+//         (R row-major, t), W*H grey bytes, W*H mask bytes, double mind, maxd,
+//         int64 n_landmarks, n_landmarks*3 doubles (world xyz)
 // output: per computed window: W*H float depth, W*H float score, W*H int32 nghbr
 #include <cstdio>
 #include <cstdlib>
@@ -35,10 +37,13 @@ int main(int argc, char** argv) {
         for (int v = 0; v < n; ++v) {
             double pose[12], mind, maxd;
             std::vector<uint8_t> img((size_t)W * H), mask((size_t)W * H);
+            int64_t nl = 0;
             if (!rd(in, pose, sizeof(pose)) || !rd(in, img.data(), img.size()) || !rd(in, mask.data(), mask.size()) ||
-                !rd(in, &mind, 8) || !rd(in, &maxd, 8))
+                !rd(in, &mind, 8) || !rd(in, &maxd, 8) || !rd(in, &nl, 8) || nl < 0)
                 return 2;
-            est.AddView(pose, pose + 9, img.data(), mask.data(), W, H, {}, maxd, mind);
+            std::vector<double> lms((size_t)nl * 3);
+            if (nl > 0 && !rd(in, lms.data(), lms.size() * sizeof(double))) return 2;
+            est.AddView(pose, pose + 9, img.data(), mask.data(), W, H, lms, maxd, mind);
             if (est.IsReadyForCompute()) {
                 pmp::DepthmapEstimatorResult r;
                 est.ComputePatchMatch(&r);

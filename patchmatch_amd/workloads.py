@@ -221,11 +221,11 @@ def run_served_stream(stream: Stream, params, specs, arrivals, max_slots=16, hbm
 
 
 def run_served(stream: Stream, params, rates, n_requests: int, seed: int):
+    """The same request list at every rate; only the arrival times change."""
     out = {}
+    specs = stream.draw(np.random.default_rng(seed), n_requests)
     for rate in rates:
-        rng = np.random.default_rng(seed + int(rate * 1000))
-        specs = stream.draw(rng, n_requests)
-        arr = _arrivals(rng, rate, n_requests)
+        arr = _arrivals(np.random.default_rng(seed + int(rate * 1000)), rate, n_requests)
         plain = run_plain_stream(stream, params, specs, arr)
         served = run_served_stream(stream, params, specs, arr)
         out[f"{rate:g}_per_s"] = {"plain": plain, "served": served,
