@@ -421,7 +421,7 @@ int pmp_write_ply(const char* path, int64_t n, const double* xyz, const uint8_t*
 
 /* Evaluate the device elementary functions (see pmp_math.h) on n inputs:
  * which 0 rcp 1 rsqrt 2 sqrt 3 asin 4 acos 5 atan2(a,b) 6 exp 7 log 8 sin2pi
- * 9 cos2pi.  Host pointers. */
+ * 9 cos2pi 10 latitude of the direction (b, a, b).  Host pointers. */
 int pmp_debug_eval_math(pmp_ctx* ctx, int32_t which, int64_t n, const float* a, const float* b, float* out);
 /* The standard normals of refinement for pixel (i, j), iteration it (W wide
  * image, params->seed), n <= 24. */

@@ -216,16 +216,35 @@ static inline float sample_f(const uint8_t* img, int W, int H, int edge, float u
     return fmaf(fy, bot - top, top);
 }
 
-/* equirect projection of a camera-frame point; PatchMatchCuda.cu:132-144 */
+/* equirect projection of a camera-frame point; PatchMatchCuda.cu:132-144
+ * (its latitude asin(y/|X|) is the angle atan2(y, sqrt(x^2+z^2)) of or_lonlat) */
 static inline void project_f(const derived_t* d, float x, float y, float z, float* u, float* v) {
-    float r2 = fmaf(x, x, fmaf(y, y, z * z));
-    r2 = or_clamp(r2, 1e-12f, 1e30f);
-    float ir = or_rsqrt_pos(r2);
-    float sy = or_clamp(y * ir, -1.0f, 1.0f);
-    float lat = or_asin(sy);
-    float lon = or_atan2(x, z);
+    float lon, lat;
+    or_lonlat(x, y, z, &lon, &lat);
     *u = fmaf(lon, d->ku, d->hu);
     *v = fmaf(lat, d->kv, d->hv);
+}
+
+/* Ray-plane intersection of tap bearing b with plane p (PatchMatchCuda.cu:117-122,
+ * |b.p| clamped to >= 1e-6 like depthmap.cc:167-173) moved into view v.  The
+ * camera-frame point is b / s, s = b.p; its view-frame position M b / s + c
+ * has the direction of sigma M b + c |s| (sigma = sign bit of s), which is all
+ * the equirect projection uses.  tap_dir_f folds sigma into b and returns
+ * max(|s|, 1e-6); view_dir_f forms the direction for view v. */
+static inline void tap_dir_f(float bx, float by, float bz, float px, float py, float pz, float* sb, float* as) {
+    float s = fmaf(pz, bz, fmaf(py, by, px * bx));
+    uint32_t sg = or_f2u(s) & 0x80000000u;
+    sb[0] = or_u2f(or_f2u(bx) ^ sg);
+    sb[1] = or_u2f(or_f2u(by) ^ sg);
+    sb[2] = or_u2f(or_f2u(bz) ^ sg);
+    *as = or_max(fabsf(s), 1e-6f);
+}
+
+static inline void view_dir_f(const float* M, const float* c, const float* sb, float as, float* x, float* y,
+                              float* z) {
+    *x = fmaf(c[0], as, fmaf(M[0], sb[0], fmaf(M[1], sb[1], M[2] * sb[2])));
+    *y = fmaf(c[1], as, fmaf(M[3], sb[0], fmaf(M[4], sb[1], M[5] * sb[2])));
+    *z = fmaf(c[2], as, fmaf(M[6], sb[0], fmaf(M[7], sb[1], M[8] * sb[2])));
 }
 
 /* ------------------------------------------------------------------ */
@@ -315,22 +334,15 @@ static float score_plane_f(const derived_t* d, const or_params* p, const or_keyf
     const uint8_t* ref = kf->ref;
     for (int t = 0; t < d->T; ++t) {
         int ii = i + d->k1[t], jj = j + d->k2[t];
-        float bx, by, bz;
+        float bx, by, bz, sb[3], as;
         bearing_f(d, ii, jj, &bx, &by, &bz);
-        /* ray-plane intersection :117-122 (|scale| clamped like depthmap.cc:167-173) */
-        float s = fmaf(pz, bz, fmaf(py, by, px * bx));
-        float as = or_clamp(fabsf(s), 1e-6f, 1e30f);
-        float rsv = or_rcp_pos(as);
-        if (s < 0.0f) rsv = -rsv;
-        float xc = bx * rsv, yc = by * rsv, zc = bz * rsv;
+        tap_dir_f(bx, by, bz, px, py, pz, sb, &as);
         float xs = (float)ref[(size_t)ii * W + jj] - rs->Ic;
         float w = d->Wc[(int)fabsf(xs)] * d->Ws[t];
         float wxc = w * (xs - rs->mx);
         for (int v = 0; v < N; ++v) {
-            const float* M = d->M[v]; const float* c = d->c[v];
-            float x = fmaf(M[0], xc, fmaf(M[1], yc, fmaf(M[2], zc, c[0])));
-            float y = fmaf(M[3], xc, fmaf(M[4], yc, fmaf(M[5], zc, c[1])));
-            float z = fmaf(M[6], xc, fmaf(M[7], yc, fmaf(M[8], zc, c[2])));
+            float x, y, z;
+            view_dir_f(d->M[v], d->c[v], sb, as, &x, &y, &z);
             float u, vv;
             project_f(d, x, y, z, &u, &vv);
             float val = sample_f(kf->views[v], W, H, p->edge_mode, u, vv);
@@ -954,7 +966,8 @@ EXPORT int or_score_field_f64(const or_keyframe* kf, const or_params* p, const f
 /* ------------------------------------------------------------------ */
 /* diagnostics for tests                                               */
 /* ------------------------------------------------------------------ */
-/* which: 0 rcp 1 rsqrt 2 sqrt 3 asin 4 acos 5 atan2(a,b) 6 exp 7 log 8 sin2pi 9 cos2pi */
+/* which: 0 rcp 1 rsqrt 2 sqrt 3 asin 4 acos 5 atan2(a,b) 6 exp 7 log 8 sin2pi 9 cos2pi
+ * 10 latitude of the direction (b, a, b) */
 EXPORT int or_eval_math(int which, int64_t n, const float* a, const float* b, float* out) {
     for (int64_t k = 0; k < n; ++k) {
         float x = a[k], r = 0.0f, s, c;
@@ -969,6 +982,7 @@ EXPORT int or_eval_math(int which, int64_t n, const float* a, const float* b, fl
             case 7: r = or_log(x); break;
             case 8: or_sincos2pi(x, &s, &c); r = s; break;
             case 9: or_sincos2pi(x, &s, &c); r = c; break;
+            case 10: or_lonlat(b[k], x, b[k], &s, &r); break; /* latitude of (b, a, b) */
             default: return -1;
         }
         out[k] = r;
@@ -994,16 +1008,10 @@ EXPORT int or_warp_point(const or_keyframe* kf, const or_params* p, int ii, int 
     derived_t d;
     if (derive(kf, p, &d)) return -1;
     if (v < 0 || v >= d.N || ii < 0 || ii >= d.H || jj < 0 || jj >= d.W) { release(&d); return -1; }
-    float bx, by, bz;
+    float bx, by, bz, sb[3], as, x, y, z;
     bearing_f(&d, ii, jj, &bx, &by, &bz);
-    float s = fmaf(plane[2], bz, fmaf(plane[1], by, plane[0] * bx));
-    float rsv = or_rcp_pos(or_clamp(fabsf(s), 1e-6f, 1e30f));
-    if (s < 0.0f) rsv = -rsv;
-    float xc = bx * rsv, yc = by * rsv, zc = bz * rsv;
-    const float* M = d.M[v]; const float* c = d.c[v];
-    float x = fmaf(M[0], xc, fmaf(M[1], yc, fmaf(M[2], zc, c[0])));
-    float y = fmaf(M[3], xc, fmaf(M[4], yc, fmaf(M[5], zc, c[1])));
-    float z = fmaf(M[6], xc, fmaf(M[7], yc, fmaf(M[8], zc, c[2])));
+    tap_dir_f(bx, by, bz, plane[0], plane[1], plane[2], sb, &as);
+    view_dir_f(d.M[v], d.c[v], sb, as, &x, &y, &z);
     project_f(&d, x, y, z, &uv[0], &uv[1]);
     release(&d);
     return 0;

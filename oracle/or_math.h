@@ -18,12 +18,17 @@
  *   rcp   : seed 0x7EF311C3 - bits(x), three Newton steps e=1-x*y, y+=y*e
  *           (max error 0.501 ulp on [1,2), exhaustive check).
  *   rsqrt : seed 0x5F375A86 - (bits(x)>>1) (C. Lomont, "Fast inverse square
- *           root", 2003), three Newton steps in residual form (<= 1 ulp).
+ *           root", 2003), one fourth-order Householder step and one Newton
+ *           step in residual form (<= 1 ulp, exhaustive over [1, 4)).
  *   asin  : |x|<=1/2: x + x^3 P(x^2); else pi/2 - 2 asin(sqrt((1-|x|)/2))
  *           (the classical reduction, e.g. Cody & Waite 1980, ch. 10).
  *   acos  : pi/2 - asin(x) for |x|<=1/2, 2 asin(sqrt((1-x)/2)) for x>1/2,
  *           pi - 2 asin(sqrt((1+x)/2)) for x<-1/2.
- *   atan2 : q = min/max in [0,1], atan(q) = q P(q^2), octant fix-up.
+ *   atan2 : q = min/max in [0,1], atan(q) = q P(q^2), octant fix-up,
+ *           sign of y.
+ *   lonlat: longitude atan2(x, z), latitude atan2(y, sqrt(x^2+z^2)) through
+ *           the same polynomial (|y| <= h: atan(|y| rsqrt(h^2)), else
+ *           pi/2 - atan(h / |y|)).
  *   exp   : Cody-Waite reduction x = k ln2 + r, |r| <= ln2/2, degree-6
  *           polynomial, scale by 2^k through the exponent field.
  *   log   : x = m 2^e, m in [sqrt(1/2), sqrt 2), log1p(m-1) = f P(f).
@@ -38,7 +43,7 @@
 
 #include <stdint.h>
 #include <string.h>
-#include <math.h> /* fmaf, floorf, fabsf only: all exact or correctly rounded */
+#include <math.h> /* fmaf, floorf, fabsf, copysignf only: all exact or correctly rounded */
 
 static inline uint32_t or_f2u(float x) { uint32_t u; memcpy(&u, &x, 4); return u; }
 static inline float or_u2f(uint32_t u) { float x; memcpy(&x, &u, 4); return x; }
@@ -70,14 +75,15 @@ static inline float or_rcp(float x) {
     return (x < 0.0f) ? -y : y;
 }
 
-/* 1/sqrt(a) for a in [1e-30, 1e30] (callers clamp) */
+/* 1/sqrt(a) for a in [1e-30, 1e30] (callers clamp): one fourth-order step
+ * y (1 + r/2 + 3r^2/8 + 5r^3/16), r = 1 - a y^2, then one Newton step */
 static inline float or_rsqrt_pos(float a) {
     float y = or_u2f(0x5F375A86u - (or_f2u(a) >> 1));
-    float ay, r;
-    ay = a * y; r = fmaf(-ay, y, 1.0f); y = fmaf(0.5f * y, r, y);
-    ay = a * y; r = fmaf(-ay, y, 1.0f); y = fmaf(0.5f * y, r, y);
-    ay = a * y; r = fmaf(-ay, y, 1.0f); y = fmaf(0.5f * y, r, y);
-    return y;
+    float r = fmaf(-(a * y), y, 1.0f);
+    y = fmaf(y * r, fmaf(fmaf(r, 0.3125f, 0.375f), r, 0.5f), y);
+    float ay = a * y;
+    r = fmaf(-ay, y, 1.0f);
+    return fmaf(0.5f * y, r, y);
 }
 
 /* sqrt(x) for x >= 0: x < 1e-30 (and NaN) -> 0, x > 1e30 -> 1e15 */
@@ -140,18 +146,56 @@ static inline float or_atan_core(float q) {
     return q * p;
 }
 
-/* atan2(y, x) for finite y, x; result in [-pi, pi]; atan2(0, 0) = 0; the
- * sign of a zero y is ignored (y = -0, x < 0 gives +pi) */
+/* min/max of two non-negative floats (no signed zeros to tell apart) with
+ * IEEE minNum/maxNum NaN handling: a NaN operand gives the other one, as
+ * v_min_f32 / v_max_f32 on the GPU */
+static inline float or_min(float a, float b) {
+    if (a != a) return b;
+    if (b != b) return a;
+    return (b < a) ? b : a;
+}
+static inline float or_max(float a, float b) {
+    if (a != a) return b;
+    if (b != b) return a;
+    return (b > a) ? b : a;
+}
+
+/* median of three (v_med3_f32): with a NaN operand, the minimum of the
+ * others (IEEE-mode min ignores a quiet NaN); no caller passes signed zeros
+ * that the median would have to order */
+static inline float or_med3(float a, float b, float c) {
+    if (a != a || b != b || c != c) return or_min(or_min(a, b), c);
+    float lo = (b < a) ? b : a, hi = (b < a) ? a : b;
+    float m = (c < hi) ? c : hi;
+    return (lo < m) ? m : lo;
+}
+
+/* atan2(y, x) for finite y, x; result in [-pi, pi] with the sign of y
+ * (atan2(+-0, x >= 0) = +-0, atan2(+-0, x < 0) = +-pi) */
 static inline float or_atan2(float y, float x) {
     float ay = fabsf(y), ax = fabsf(x);
-    float mn = (ay < ax) ? ay : ax;
-    float mx = (ay < ax) ? ax : ay;
-    float q = (mx >= 1e-30f) ? mn * or_rcp_pos(or_clamp(mx, 1e-30f, 1e30f)) : 0.0f;
-    q = (q <= 1.0f) ? q : 1.0f;
+    float mn = or_med3(ay, ax, 0.0f);                              /* min of two non-negatives */
+    float mx = or_med3(or_med3(ay, ax, 1e30f), 1e-30f, 1e30f);     /* max, clamped */
+    float q = or_med3(mn * or_rcp_pos(mx), 0.0f, 1.0f);
     float a = or_atan_core(q);
     if (ay > ax) a = OR_PIO2_F - a;
     if (x < 0.0f) a = OR_PI_F - a;
-    return (y < 0.0f) ? -a : a;
+    return copysignf(a, y);
+}
+
+/* longitude atan2(x, z) and latitude atan2(y, h), h = sqrt(x^2 + z^2), of a
+ * direction with finite components; h^2 clamped to >= 1e-24 */
+static inline void or_lonlat(float x, float y, float z, float* lon, float* lat) {
+    float h2 = or_max(fmaf(x, x, z * z), 1e-24f);
+    float ih = or_rsqrt_pos(h2);
+    float ay = fabsf(y);
+    float t = ay * ih;
+    int big = t > 1.0f;
+    if (big) t = (h2 * ih) * or_rcp_pos(or_clamp(ay, 1e-30f, 1e30f));
+    float a = or_atan_core(or_med3(t, 0.0f, 1.0f));
+    if (big) a = OR_PIO2_F - a;
+    *lat = copysignf(a, y);
+    *lon = or_atan2(x, z);
 }
 
 /* exp(x); x <= -87.3 -> 0, x >= 88 -> e^88 clamp; NaN -> e^-87.3 path (0) */

@@ -164,24 +164,27 @@ int get_tables(pmp_ctx* ctx, int W, int H, float sigma_color, Tables& t) {
     return PMP_OK;
 }
 
-// (x,y) (x+1,y) (x,y+1) (x+1,y+1) in bytes 0..3; x+1 of the last column wraps
-// to 0 (longitude wrap) or repeats the last column; y+1 of the last row repeats it.
+// (x,y) (x+1,y) (x,y+1) (x+1,y+1) in bytes 0..3 at word y*(W+1) + x, for
+// x in [0, W]: column x = W is column 0 again (longitude wrap: a sample at
+// u = W reads it with no index wrap) or column W-1 (clamp); x+1 of the last
+// column wraps to 0 or repeats the last column; y+1 of the last row repeats it.
 __global__ void k_pack(const uint8_t* __restrict__ g, uint32_t* __restrict__ q, int W, int H, int wrap) {
-    int x = blockIdx.x * blockDim.x + threadIdx.x;
+    int xo = blockIdx.x * blockDim.x + threadIdx.x;
     int y = blockIdx.y;
-    if (x >= W || y >= H) return;
+    if (xo > W || y >= H) return;
+    int x = xo < W ? xo : (wrap ? 0 : W - 1);
     int x1 = x + 1 < W ? x + 1 : (wrap ? 0 : W - 1);
     int y1 = y + 1 < H ? y + 1 : H - 1;
     const uint8_t* r0 = g + (size_t)y * W;
     const uint8_t* r1 = g + (size_t)y1 * W;
-    q[(size_t)y * W + x] = (uint32_t)r0[x] | ((uint32_t)r0[x1] << 8) | ((uint32_t)r1[x] << 16) |
-                           ((uint32_t)r1[x1] << 24);
+    q[(size_t)y * (W + 1) + xo] = (uint32_t)r0[x] | ((uint32_t)r0[x1] << 8) | ((uint32_t)r1[x] << 16) |
+                                  ((uint32_t)r1[x1] << 24);
 }
 
 int ensure_quads(pmp_ctx* ctx, Image& im, int wrap, hipStream_t stream) {
     if (im.quads && im.quads_wrap == wrap) return PMP_OK;
-    if (!im.quads) PMP_HIP(ctx, hipMalloc(&im.quads, im.bytes * sizeof(uint32_t)));
-    dim3 blk(256), grd((im.W + 255) / 256, im.H);
+    if (!im.quads) PMP_HIP(ctx, hipMalloc(&im.quads, Image::quad_bytes(im.W, im.H)));
+    dim3 blk(256), grd((im.W + 1 + 255) / 256, im.H);
     hipLaunchKernelGGL(k_pack, grd, blk, 0, stream, im.gray, im.quads, im.W, im.H, wrap);
     PMP_HIP(ctx, hipGetLastError());
     im.quads_wrap = wrap;
@@ -384,7 +387,7 @@ int pmp_image_drop(pmp_ctx* ctx, uint64_t id) {
 int pmp_image_pool_stats(const pmp_ctx* ctx, int64_t* count, int64_t* bytes) {
     if (!ctx) return PMP_EINVAL;
     int64_t b = 0;
-    for (auto& kv : ctx->images) b += (int64_t)kv.second.bytes * (kv.second.quads ? 5 : 1);
+    for (auto& kv : ctx->images) b += (int64_t)kv.second.pool_bytes();
     if (count) *count = (int64_t)ctx->images.size();
     if (bytes) *bytes = b;
     return PMP_OK;

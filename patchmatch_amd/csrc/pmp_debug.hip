@@ -25,6 +25,7 @@ __global__ void k_eval_math(int which, int n, const float* __restrict__ a, const
         case 7: r = log_(x); break;
         case 8: sincos2pi(x, &s, &c); r = s; break;
         case 9: sincos2pi(x, &s, &c); r = c; break;
+        case 10: lonlat(b[k], x, b[k], s, r); break;   // latitude of (b, a, b)
         default: break;
     }
     out[k] = r;
@@ -59,7 +60,7 @@ using namespace pmp;
 extern "C" {
 
 int pmp_debug_eval_math(pmp_ctx* ctx, int32_t which, int64_t n, const float* a, const float* b, float* out) {
-    if (!ctx || !a || !out || n < 0 || n > (1ll << 28) || which < 0 || which > 9) return PMP_EINVAL;
+    if (!ctx || !a || !out || n < 0 || n > (1ll << 28) || which < 0 || which > 10) return PMP_EINVAL;
     if (n == 0) return PMP_OK;
     int rc = ensure_dbg(ctx, 3 * (size_t)n);
     if (rc) return rc;

@@ -30,7 +30,7 @@ struct KfArgs {
     float c[kMaxViews][3];                // t_n - M t_ref
     float ku, hu, kv, hv;                 // u = lon*ku + hu, v = asin*kv + hv
     float umax, vmax;                     // sample clamps: W (wrap) or W-1, and H-1
-    uint32_t view_bytes;                  // bytes of one packed view (4*H*W)
+    uint32_t view_bytes;                  // bytes of one packed view (4*H*(W+1))
     float var_floor, fold_agree, inv_k;
     float depth_penalty, plane_penalty;
     float dr[kMaxRefine], nr[kMaxRefine];
@@ -59,10 +59,15 @@ struct Image {
     int W = 0, H = 0;
     uint8_t* gray = nullptr;          // H*W
     size_t bytes = 0;
-    uint32_t* quads = nullptr;        // H*W words: (x,y) (x+1,y) (x,y+1) (x+1,y+1), built on first use
+    uint32_t* quads = nullptr;        // H*(W+1) words: (x,y) (x+1,y) (x,y+1) (x+1,y+1), built on first use
     int quads_wrap = -1;              // column x+1 of the last column: 1 wraps to 0, 0 clamps; -1 not built
     uint64_t last_use = 0;            // pool clock of the last request that read it (LRU)
     int pins = 0;                     // requests in flight that read it (never evicted while > 0)
+    // device bytes: the grey image and, once built, its packed words
+    size_t pool_bytes() const { return bytes + (quads ? quad_bytes(W, H) : 0); }
+    // packed words of a W x H image: rows of W + 1 words, the extra column
+    // x = W repeating column 0 (wrap) or column W-1 (clamp)
+    static size_t quad_bytes(int W, int H) { return (size_t)(W + 1) * H * sizeof(uint32_t); }
 };
 
 struct QueueState;   // pmp_queue.h

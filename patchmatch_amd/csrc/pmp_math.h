@@ -1,7 +1,8 @@
 // pmp_math.h -- device elementary functions and random streams.
 //
-// Built only from IEEE-754 operations that are correctly rounded on gfx950
-// (v_add/v_mul/v_fma_f32, compares, v_floor_f32, integer and bit ops): no
+// Built only from IEEE-754 operations that are correctly rounded or exact on
+// gfx950 (v_add/v_mul/v_fma_f32, compares, min/max/med3, v_floor_f32,
+// v_fract_f32 of non-negative values, copysign, integer and bit ops): no
 // division, no hardware sqrt/rcp/rsq approximations, no transcendental
 // instructions.  With -ffp-contract=off the compiler emits exactly the fma /
 // mul / add sequence written here, so results are reproducible bit for bit
@@ -9,12 +10,14 @@
 //
 // Methods (published; coefficients fitted by gen_coeffs.py in this folder):
 //   rcp    seed 0x7EF311C3 - bits(x), three Newton steps (<= 0.501 ulp)
-//   rsqrt  seed 0x5F375A86 - (bits(x) >> 1) (Lomont 2003), three Newton
-//          steps in residual form (<= 1 ulp)
+//   rsqrt  seed 0x5F375A86 - (bits(x) >> 1) (Lomont 2003), one
+//          fourth-order Householder step, one Newton step (<= 1 ulp)
 //   asin   |x| <= 1/2: x + x^3 P(x^2); else pi/2 - 2 asin(sqrt((1-|x|)/2))
 //          (Cody & Waite, Software Manual for the Elementary Functions, 1980)
 //   acos   from the same core polynomial, three ranges
 //   atan2  q = min/max, atan(q) = q P(q^2) on [0, 1], octant fix-up
+//   lonlat longitude atan2(x, z), latitude atan2(y, sqrt(x^2 + z^2)) through
+//          the same atan polynomial
 //   exp    Cody-Waite reduction by ln2, degree-6 polynomial, 2^k by exponent bits
 //   log    mantissa in [sqrt(1/2), sqrt 2), log1p(f) = f P(f)
 //   sincos2pi  quadrant from floor(4u), sin/cos(pi f / 2) in f^2
@@ -60,14 +63,15 @@ PMP_DF float rcp(float x) {
     return (x < 0.0f) ? -y : y;
 }
 
-// 1/sqrt(a), a in [1e-30, 1e30]
+// 1/sqrt(a), a in [1e-30, 1e30]: one fourth-order step
+// y (1 + r/2 + 3r^2/8 + 5r^3/16), r = 1 - a y^2, then one Newton step
 PMP_DF float rsqrt_pos(float a) {
     float y = u2f(0x5F375A86u - (f2u(a) >> 1));
-    float ay, r;
-    ay = a * y; r = fma_(-ay, y, 1.0f); y = fma_(0.5f * y, r, y);
-    ay = a * y; r = fma_(-ay, y, 1.0f); y = fma_(0.5f * y, r, y);
-    ay = a * y; r = fma_(-ay, y, 1.0f); y = fma_(0.5f * y, r, y);
-    return y;
+    float r = fma_(-(a * y), y, 1.0f);
+    y = fma_(y * r, fma_(fma_(r, 0.3125f, 0.375f), r, 0.5f), y);
+    const float ay = a * y;
+    r = fma_(-ay, y, 1.0f);
+    return fma_(0.5f * y, r, y);
 }
 
 // sqrt(x): x < 1e-30 (and NaN) -> 0, x > 1e30 -> 1e15
@@ -126,18 +130,42 @@ PMP_DF float atan_core(float q) {
     return q * p;
 }
 
-// atan2(y, x) for finite arguments, in [-pi, pi]; atan2(0, 0) = 0
+// min/max of two non-negative floats (no signed zeros to tell apart); a NaN
+// operand gives the other one (v_min_f32 / v_max_f32, C fminf / fmaxf)
+PMP_DF float min_(float a, float b) { return __builtin_fminf(a, b); }
+PMP_DF float max_(float a, float b) { return __builtin_fmaxf(a, b); }
+// median of three (v_med3_f32; with a NaN operand the minimum of the others).
+// med3(a, b, 0) is min(a, b) and med3(a, b, K) is max(a, b) for a, b in
+// [0, K]; unlike fminf/fmaxf it needs no quieting of its operands.
+PMP_DF float med3_(float a, float b, float c) { return __builtin_amdgcn_fmed3f(a, b, c); }
+
+// atan2(y, x) for finite arguments, in [-pi, pi], with the sign of y
+// (atan2(+-0, x >= 0) = +-0, atan2(+-0, x < 0) = +-pi)
 PMP_DF float atan2_(float y, float x) {
-    float ay = abs_(y), ax = abs_(x);
-    float mn = (ay < ax) ? ay : ax;
-    float mx = (ay < ax) ? ax : ay;
-    float q = mn * rcp_pos(clampf(mx, 1e-30f, 1e30f));
-    q = (mx >= 1e-30f) ? q : 0.0f;
-    q = (q <= 1.0f) ? q : 1.0f;
+    const float ay = abs_(y), ax = abs_(x);
+    const float mn = med3_(ay, ax, 0.0f);                           // min of two non-negatives
+    const float mx = med3_(med3_(ay, ax, 1e30f), 1e-30f, 1e30f);    // max, clamped
+    const float q = med3_(mn * rcp_pos(mx), 0.0f, 1.0f);
     float a = atan_core(q);
     if (ay > ax) a = kPiO2 - a;
     if (x < 0.0f) a = kPi - a;
-    return (y < 0.0f) ? -a : a;
+    return __builtin_copysignf(a, y);
+}
+
+// Longitude atan2(x, z) and latitude atan2(y, h), h = sqrt(x^2 + z^2), of a
+// direction (x, y, z) with finite components.  |y| <= h: atan(|y| / h) with
+// 1/h from rsqrt; |y| > h: pi/2 - atan(h / |y|).  h^2 is clamped to >= 1e-24.
+PMP_DF void lonlat(float x, float y, float z, float& lon, float& lat) {
+    const float h2 = max_(fma_(x, x, z * z), 1e-24f);
+    const float ih = rsqrt_pos(h2);
+    const float ay = abs_(y);
+    float t = ay * ih;
+    const bool big = t > 1.0f;
+    if (big) t = (h2 * ih) * rcp_pos(clampf(ay, 1e-30f, 1e30f));
+    float a = atan_core(med3_(t, 0.0f, 1.0f));
+    if (big) a = kPiO2 - a;
+    lat = __builtin_copysignf(a, y);
+    lon = atan2_(x, z);
 }
 
 // exp(x); x <= -87.3 or NaN -> 0; x >= 88 clamped

@@ -8,7 +8,9 @@
 // the canonical FP32 sequence of this project (identical, operation for
 // operation, to the host restatement):
 //   * bearings from separable row/column tables;
-//   * one fused rigid transform per view (M, c);
+//   * per view, the direction sigma M b + c |b.p| of the plane point (no
+//     per-tap reciprocal: the projection only needs the direction);
+//   * longitude and latitude through one atan polynomial (lonlat);
 //   * weights Wc[|dI|] * Ws[tap] from tables;
 //   * shifted moments (reference taps centred on their weighted mean,
 //     neighbour samples shifted by the view's first-tap sample);
@@ -99,11 +101,16 @@ PMP_DF float ncc(const RefStat& rs, float floor_, float Swy, float Swyy, float S
 
 // Neighbour sample at (u, v); texel k at integer k (PatchMatchCuda.cu:146 looks
 // up (u+0.5, v+0.5) with texel centres at k+0.5).  The packed words of a view
-// are read with a uniform 64-bit base and a 32-bit byte offset; the clamps
-// below keep every offset inside the view (a NaN coordinate clamps to 0).
+// (rows of W + 1 words) are read with a uniform 64-bit base and a 32-bit byte
+// offset; the clamps below keep every offset inside the view (a NaN
+// coordinate clamps to 0).
 //   kZero: CPU semantics (depthmap.cc:36-38), 0 outside [0,W-1) x [0,H-1);
-//   else:  u clamped to [0, umax] (umax = W with longitude wrap packed into
-//          the words, W-1 for clamp), v clamped to [0, H-1].
+//   else:  u clamped to [0, umax] (umax = W with longitude wrap: column W of
+//          the words repeats column 0; W-1 for clamp), v clamped to [0, H-1].
+//   The instruction choices are exact for the clamped coordinates (uu, vv >= 0,
+//   finite): truncation equals floor, fract(x) = x - floor(x) is exact, med3
+//   with lo <= hi is the clamp (a NaN coordinate gives lo, as clampf), and
+//   the 24-bit product covers rows and row strides below 2^24.
 template <bool kZero>
 PMP_DF float sample(const KfArgs& a, const uint32_t* __restrict__ base, float u, float v) {
     const int W = a.W;
@@ -114,14 +121,13 @@ PMP_DF float sample(const KfArgs& a, const uint32_t* __restrict__ base, float u,
         uu = inside ? u : 0.0f;
         vv = inside ? v : 0.0f;
     } else {
-        uu = clampf(u, 0.0f, a.umax);
-        vv = clampf(v, 0.0f, a.vmax);
+        uu = __builtin_amdgcn_fmed3f(u, 0.0f, a.umax);
+        vv = __builtin_amdgcn_fmed3f(v, 0.0f, a.vmax);
     }
-    const float x0f = floor_(uu), y0f = floor_(vv);
-    const float fx = uu - x0f, fy = vv - y0f;
-    int x0 = (int)x0f;
-    if constexpr (!kZero) x0 = (x0 >= W) ? x0 - W : x0;
-    const uint32_t idx = (uint32_t)((int)y0f * W + x0);
+    const float fx = __builtin_amdgcn_fractf(uu), fy = __builtin_amdgcn_fractf(vv);
+    const uint32_t x0 = (uint32_t)(int)uu;
+    const uint32_t y0 = (uint32_t)(int)vv;
+    const uint32_t idx = __umul24(y0, (uint32_t)(W + 1)) + x0;
     const uint32_t word = *(const uint32_t*)((const char*)base + idx * 4u);
     const float b0 = (float)(word & 0xFFu);
     const float b1 = (float)((word >> 8) & 0xFFu);
@@ -134,16 +140,22 @@ PMP_DF float sample(const KfArgs& a, const uint32_t* __restrict__ base, float u,
     return val;
 }
 
-// Equirectangular projection (PatchMatchCuda.cu:132-144).
-PMP_DF void project(const KfArgs& a, float x, float y, float z, float& u, float& v) {
-    float r2 = fma_(x, x, fma_(y, y, z * z));
-    r2 = clampf(r2, 1e-12f, 1e30f);
-    float ir = rsqrt_pos(r2);
-    float sy = clampf(y * ir, -1.0f, 1.0f);
-    float lat = asin_(sy);
-    float lon = atan2_(x, z);
-    u = fma_(lon, a.ku, a.hu);
-    v = fma_(lat, a.kv, a.hv);
+// Equirectangular projection (PatchMatchCuda.cu:132-144): the latitude
+// asin(y / |X|) there is the same angle as atan2(y, sqrt(x^2 + z^2)) here.
+// hu, hv are a.hu, a.hv (callers may pass copies held in vector registers).
+PMP_DF void project(const KfArgs& a, float hu, float hv, float x, float y, float z, float& u, float& v) {
+    float lon, lat;
+    lonlat(x, y, z, lon, lat);
+    u = fma_(lon, a.ku, hu);
+    v = fma_(lat, a.kv, hv);
+}
+
+// A uniform value copied into a vector register, so that an fma with two
+// other uniform operands needs no move of its own in a loop.
+PMP_DF float in_vgpr(float s) {
+    float v;
+    asm("v_mov_b32 %0, %1" : "=v"(v) : "s"(s));
+    return v;
 }
 
 template <int NV>
@@ -227,9 +239,39 @@ PMP_DF float aggregate_rt(const KfArgs& a, const float* sv, int ld) {
     return s;
 }
 
+// Ray-plane intersection of tap bearing b with plane p (PatchMatchCuda.cu:117-122,
+// |b.p| clamped to >= 1e-6) moved into a view: the camera-frame point b / s,
+// s = b.p, lands at M b / s + c, whose direction sigma M b + c |s| (sigma =
+// sign bit of s) is all the equirectangular projection uses.  tap_dir folds
+// sigma into b and returns max(|s|, 1e-6); view_sample forms the direction
+// for view v, projects and samples it.
+PMP_DF void tap_dir(float bx, float by, float bz, float px, float py, float pz, float& sbx, float& sby, float& sbz,
+                    float& as) {
+    const float s = fma_(pz, bz, fma_(py, by, px * bx));
+    const uint32_t sg = f2u(s) & 0x80000000u;
+    sbx = u2f(f2u(bx) ^ sg);
+    sby = u2f(f2u(by) ^ sg);
+    sbz = u2f(f2u(bz) ^ sg);
+    as = max_(abs_(s), 1e-6f);
+}
+
+template <bool kZero>
+PMP_DF float view_sample(const KfArgs& a, float hu, float hv, const uint32_t* base, int v, float sbx, float sby,
+                         float sbz, float as) {
+    const float x = fma_(a.c[v][0], as, fma_(a.M[v][0], sbx, fma_(a.M[v][1], sby, a.M[v][2] * sbz)));
+    const float y = fma_(a.c[v][1], as, fma_(a.M[v][3], sbx, fma_(a.M[v][4], sby, a.M[v][5] * sbz)));
+    const float z = fma_(a.c[v][2], as, fma_(a.M[v][6], sbx, fma_(a.M[v][7], sby, a.M[v][8] * sbz)));
+    float u, vv;
+    project(a, hu, hv, x, y, z, u, vv);
+    return sample<kZero>(a, base, u, vv);
+}
+
 // Moments of views [v0, v0+G) (G <= 3 live views keep the per-view
 // transforms in scalar registers; v0 is uniform).  Per view the taps are
 // accumulated in tap order, exactly as in a single pass over all views.
+// The first tap's sample is the view's shift, so that tap adds exact zeros
+// to the three moments (yd = 0, w >= 0): it is sampled before the loop and
+// the loop starts at the second tap.
 template <int G, bool kZero>
 PMP_DF void view_group(const KfArgs& a, const float* sWc, const RefStat& rs, int i, int j, float px, float py,
                        float pz, int v0, float (&out)[G]) {
@@ -238,33 +280,36 @@ PMP_DF void view_group(const KfArgs& a, const float* sWc, const RefStat& rs, int
 #pragma unroll
     for (int g = 0; g < G; ++g) base[g] = a.vq[v0 + g];
     float acc0[G], acc1[G], acc2[G], shift[G];
+    const float hu = in_vgpr(a.hu), hv = in_vgpr(a.hv);
+    {
+        const float2 rt = a.rowtab[i - a.half];
+        const float2 ct = a.coltab[j - a.half];
+        float sbx, sby, sbz, as;
+        tap_dir(rt.x * ct.x, -rt.y, rt.x * ct.y, px, py, pz, sbx, sby, sbz, as);
 #pragma unroll
-    for (int g = 0; g < G; ++g) { acc0[g] = 0.0f; acc1[g] = 0.0f; acc2[g] = 0.0f; shift[g] = 0.0f; }
-    int t = 0;
-    for (int k1 = -a.half; k1 <= a.half; k1 += a.stride) {
+        for (int g = 0; g < G; ++g) {
+            shift[g] = view_sample<kZero>(a, hu, hv, base[g], v0 + g, sbx, sby, sbz, as);
+            acc0[g] = 0.0f;
+            acc1[g] = 0.0f;
+            acc2[g] = 0.0f;
+        }
+    }
+    int t = 1;
+    int k2 = -a.half + a.stride;
+    for (int k1 = -a.half; k1 <= a.half; k1 += a.stride, k2 = -a.half) {
         const int ii = i + k1;
         const float2 rt = a.rowtab[ii];
         const uint8_t* row = a.ref + (size_t)ii * W + j;
-        for (int k2 = -a.half; k2 <= a.half; k2 += a.stride, ++t) {
+        for (; k2 <= a.half; k2 += a.stride, ++t) {
             const float2 ct = a.coltab[j + k2];
-            const float bx = rt.x * ct.x, by = -rt.y, bz = rt.x * ct.y;
-            const float s = fma_(pz, bz, fma_(py, by, px * bx));
-            float rsv = rcp_pos(clampf(abs_(s), 1e-6f, 1e30f));
-            if (s < 0.0f) rsv = -rsv;
-            const float xc = bx * rsv, yc = by * rsv, zc = bz * rsv;
+            float sbx, sby, sbz, as;
+            tap_dir(rt.x * ct.x, -rt.y, rt.x * ct.y, px, py, pz, sbx, sby, sbz, as);
             const float xs = (float)row[k2] - rs.Ic;
             const float w = sWc[(int)abs_(xs)] * a.Ws[t];
             const float wxc = w * (xs - rs.mx);
 #pragma unroll
             for (int g = 0; g < G; ++g) {
-                const int v = v0 + g;
-                const float x = fma_(a.M[v][0], xc, fma_(a.M[v][1], yc, fma_(a.M[v][2], zc, a.c[v][0])));
-                const float y = fma_(a.M[v][3], xc, fma_(a.M[v][4], yc, fma_(a.M[v][5], zc, a.c[v][1])));
-                const float z = fma_(a.M[v][6], xc, fma_(a.M[v][7], yc, fma_(a.M[v][8], zc, a.c[v][2])));
-                float u, vv;
-                project(a, x, y, z, u, vv);
-                const float val = sample<kZero>(a, base[g], u, vv);
-                shift[g] = (t == 0) ? val : shift[g];
+                const float val = view_sample<kZero>(a, hu, hv, base[g], v0 + g, sbx, sby, sbz, as);
                 const float yd = val - shift[g];
                 const float tw = w * yd;
                 acc0[g] = acc0[g] + tw;

@@ -114,7 +114,7 @@ int srv_error(pmp_server* s, int code, const std::string& msg) {
     return code;
 }
 
-size_t image_bytes(const Image& im) { return im.bytes * (im.quads ? 5 : 1); }
+size_t image_bytes(const Image& im) { return im.pool_bytes(); }
 
 // Evicts unpinned pool images, least recently used first, until `need` more
 // bytes fit in the budget.  Caller holds mu.  Returns false if they cannot.
@@ -412,12 +412,12 @@ int pool_acquire(pmp_server* s, uint64_t id, const uint8_t* pixels, int W, int H
             r->pins.push_back(id);
             if (!pack || it->second.quads) return PMP_OK;
             gray = it->second.gray;   // pinned: stays while we pack it
-            int rc = reserve(s, lk, (int64_t)bytes * 4);
+            int rc = reserve(s, lk, (int64_t)Image::quad_bytes(W, H));
             if (rc) return rc;
         } else {
             if (!pixels)
                 return srv_error(s, PMP_ENOTFOUND, "image " + std::to_string(id) + " is not pooled and no pixels given");
-            int rc = reserve(s, lk, (int64_t)bytes * (pack ? 5 : 1));
+            int rc = reserve(s, lk, (int64_t)(bytes + (pack ? Image::quad_bytes(W, H) : 0)));
             if (rc) return rc;
             fresh = true;
         }
@@ -443,7 +443,7 @@ int pool_acquire(pmp_server* s, uint64_t id, const uint8_t* pixels, int W, int H
     if (rc != PMP_OK) {
         if (fresh && im.gray) (void)hipFree(im.gray);
         if (im.quads) (void)hipFree(im.quads);
-        s->used -= (int64_t)bytes * (fresh ? (pack ? 5 : 1) : 4);
+        s->used -= (int64_t)((fresh ? bytes : 0) + (pack ? Image::quad_bytes(W, H) : 0));
         return srv_error(s, PMP_EDEVICE, "upload of image " + std::to_string(id) + " failed");
     }
     if (fresh) {

@@ -44,12 +44,17 @@ MATH_CASES = {
 }
 
 
-@pytest.mark.parametrize("name", sorted(MATH_CASES) + ["atan2"])
+@pytest.mark.parametrize("name", sorted(MATH_CASES) + ["atan2", "lat"])
 def test_math_bitwise(gpu_ctx, name):
     rng = np.random.default_rng(11)
     if name == "atan2":
-        a = np.concatenate([rng.normal(0, 10, 300000), [0.0, 0.0, -1.0, 1e-35, 5.0]]).astype(np.float32)
-        b = np.concatenate([rng.normal(0, 10, 300000), [0.0, -1.0, 0.0, 1e-35, -5.0]]).astype(np.float32)
+        a = np.concatenate([rng.normal(0, 10, 300000), [0.0, 0.0, -1.0, 1e-35, 5.0, -0.0, -0.0]]).astype(np.float32)
+        b = np.concatenate([rng.normal(0, 10, 300000), [0.0, -1.0, 0.0, 1e-35, -5.0, -1.0, 1.0]]).astype(np.float32)
+    elif name == "lat":    # latitude of the direction (b, a, b)
+        a = np.concatenate([rng.normal(0, 10, 300000), rng.normal(0, 1e-3, 1000),
+                            [0.0, -0.0, 1.0, -1.0, 1e-20, 7.0]]).astype(np.float32)
+        b = np.concatenate([rng.normal(0, 10, 300000), rng.normal(0, 10, 1000),
+                            [0.0, 1.0, 0.0, 0.0, 1e-20, 7.0 / np.sqrt(2)]]).astype(np.float32)
     else:
         a = MATH_CASES[name](rng).astype(np.float32)
         b = None

@@ -76,6 +76,23 @@ def test_atan2_vs_libm():
     y = or_py.eval_math("atan2", a, b)
     assert ulp_err(y, np.arctan2(a.astype(np.float64), b.astype(np.float64))).max() <= 3.0
     assert or_py.eval_math("atan2", np.float32([0.0]), np.float32([0.0]))[0] == 0.0
+    # the sign of a zero y is kept, as C atan2: atan2(-0, -1) = -pi, atan2(-0, 1) = -0
+    z = or_py.eval_math("atan2", np.float32([-0.0, -0.0, 0.0]), np.float32([-1.0, 1.0, -1.0]))
+    assert z[0] == np.float32(-np.pi) and z[1] == 0.0 and np.signbit(z[1]) and z[2] == np.float32(np.pi)
+
+
+def test_latitude_vs_libm():
+    """Latitude of the direction (x, y, z) = (b, a, b): atan2(a, sqrt(2) |b|),
+    over both branches (|y| below and above the horizontal norm) and the poles."""
+    rng = np.random.default_rng(3)
+    a = np.concatenate([rng.normal(0, 5, 200000), rng.normal(0, 1e-3, 1000), [1.0, -1.0, 0.0, 3.0]])
+    b = np.concatenate([rng.normal(0, 5, 200000), rng.normal(0, 5, 1000), [0.0, 0.0, 1.0, 3.0 / np.sqrt(2)]])
+    a, b = a.astype(np.float32), b.astype(np.float32)
+    lat = or_py.eval_math("lat", a, b)
+    h = np.sqrt(2.0) * np.abs(b.astype(np.float64))
+    exact = np.arctan2(a.astype(np.float64), h)
+    assert ulp_err(lat, exact).max() <= 3.0
+    assert lat[-4] == np.float32(np.pi / 2) and lat[-3] == -np.float32(np.pi / 2) and lat[-2] == 0.0
 
 
 def test_log_and_sincos():
