@@ -295,7 +295,10 @@ typedef struct pmp_server_opts {
     int64_t hbm_budget;         /* device bytes for pooled images (5 B/pixel) and keyframe maps
                                    (84 B/pixel); 0: 64 GiB */
     int32_t edge_mode;          /* PMP_EDGE_* of every request (neighbour packing is shared) */
-    int32_t reserved;
+    int32_t inflight_kpx;       /* admission budget in thousands of pixels: a queued request is
+                                   admitted (oldest first) only while the keyframes in flight plus
+                                   it stay within the budget, or when nothing is in flight; 0: no
+                                   budget (only max_slots limits) */
     pmp_batch_config config;    /* tile_rows, blocks_per_cu, xcd_bands of the step kernels */
 } pmp_server_opts;
 

@@ -168,19 +168,21 @@ void give_host(pmp_server* s, HostBuf& b) {
     b = HostBuf();
 }
 
-// Admission of queued requests into free slots (caller holds mu).
+// Admission of queued requests into free slots, oldest first, within the
+// in-flight pixel budget (caller holds mu).
 void admit(pmp_server* s) {
     while (!s->queue.empty() && !s->paused) {
         int slot = -1;
-        for (int k = 0; k < s->max_slots; ++k)
-            if (!s->slot_req[k]) {
-                slot = k;
-                break;
-            }
+        int64_t inflight = 0;
+        for (int k = 0; k < s->max_slots; ++k) {
+            if (s->slot_req[k]) inflight += (int64_t)s->slot_req[k]->W * s->slot_req[k]->H;
+            else if (slot < 0) slot = k;
+        }
         if (slot < 0) return;
         Request* r = s->queue.front();
         SlotBuf& b = s->slot_buf[slot];
         const size_t P = (size_t)r->W * r->H;
+        if (s->opts.inflight_kpx > 0 && inflight > 0 && inflight + (int64_t)P > 1000ll * s->opts.inflight_kpx) return;
         if (P > b.cap) {
             const int64_t grow = (int64_t)((P - b.cap) * SlotBuf::kBytesPerPixel);
             if (!make_room(s, grow)) {
@@ -475,6 +477,7 @@ int pmp_server_create(pmp_server** out, const pmp_server_opts* opts) {
     if (o.hbm_budget == 0) o.hbm_budget = 64ll << 30;
     if (o.config.tile_rows == 0) o.config = pmp_batch_config{8, 0, 1, 1};
     if (o.max_slots < 1 || o.max_slots > kMaxSlots || o.hbm_budget < 0 || o.edge_mode < 0 || o.edge_mode > 2 ||
+        o.inflight_kpx < 0 ||
         (o.config.tile_rows != 2 && o.config.tile_rows != 4 && o.config.tile_rows != 8))
         return PMP_EINVAL;
     pmp_server* s = new (std::nothrow) pmp_server();

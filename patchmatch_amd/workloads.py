@@ -168,10 +168,10 @@ def run_plain_stream(stream: Stream, params, specs, arrivals):
     return _summary(lat, arrivals[0], max(done), len(specs))
 
 
-def run_served_stream(stream: Stream, params, specs, arrivals, max_slots=16, hbm_budget=64 << 30):
+def run_served_stream(stream: Stream, params, specs, arrivals, max_slots=16, hbm_budget=64 << 30, inflight_kpx=0):
     """Requests submitted at their arrival times to a pmp_server."""
     from ._lib import Server
-    srv = Server(0, max_slots=max_slots, hbm_budget=hbm_budget, edge_mode=params.edge_mode)
+    srv = Server(0, max_slots=max_slots, hbm_budget=hbm_budget, edge_mode=params.edge_mode, inflight_kpx=inflight_kpx)
     # warm the kernels once (not timed)
     kf = stream.keyframe(specs[0])
     t = srv.submit(params, kf["ref_id"], kf["ref_pose"], kf["view_ids"], kf["view_poses"], kf["shape"],

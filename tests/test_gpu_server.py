@@ -92,6 +92,36 @@ def test_server_same_alone_and_in_the_mix(gpu_ctx, slots):
         assert st["max_in_flight"] == min(slots, len(kfs))
 
 
+def test_server_inflight_pixel_budget(gpu_ctx):
+    """An admission budget of 120 thousand pixels: requests are admitted oldest
+    first while the keyframes in flight stay within it (a larger one alone),
+    results unchanged."""
+    kfs = _keyframes()
+    want = _plain(gpu_ctx, kfs)
+    budget_px = 120_000
+    with Server(0, max_slots=16, inflight_kpx=budget_px // 1000) as srv:
+        srv.pause(True)
+        tickets = [_submit(srv, kfs, k) for k in range(len(kfs))]
+        srv.pause(False)
+        infos = []
+        for k, t in enumerate(tickets):
+            assert srv.wait(t, 60000), f"request {k} not done in 60 s"
+            got = srv.fetch(t)
+            _check(got, want[k], f"budget request {k}")
+            infos.append(got[4])
+        st = srv.stats()
+        print(f"[server budget] {st}", flush=True)
+        assert st["completed"] == len(kfs) and 1 < st["max_in_flight"] < len(kfs)
+    # admission is oldest first: admit times are non-decreasing in submission order
+    adm = [i["admit_ms"] for i in infos]
+    assert adm == sorted(adm)
+    # at every admission, the keyframes in flight (admitted, not yet done) fit the budget
+    for k, i in enumerate(infos):
+        flying = [j for j in range(len(infos)) if infos[j]["admit_ms"] <= i["admit_ms"] < infos[j]["done_ms"]]
+        px = sum(kfs[j].W * kfs[j].H for j in flying)
+        assert len(flying) == 1 or px <= budget_px, (k, flying, px)
+
+
 def test_server_staggered_arrivals(gpu_ctx):
     """Requests submitted from another thread while earlier ones run."""
     kfs = _keyframes()
