@@ -80,7 +80,6 @@ int build_args(pmp_ctx* ctx, const pmp_params* p, uint64_t ref_id, int N, const 
         if (rc) return rc;
         a.vq[v] = iv->second.quads;
     }
-    a.view_bytes = (uint32_t)Image::quad_bytes(a.W, a.H);
     a.rowtab = tb.row; a.coltab = tb.col; a.Wc = tb.wc;
     derive_poses(ref_pose, a.N, view_poses, a.M, a.c);
     a.ku = (float)((double)a.W / (2.0 * M_PI));

@@ -30,7 +30,6 @@ struct KfArgs {
     float c[kMaxViews][3];                // t_n - M t_ref
     float ku, hu, kv, hv;                 // u = lon*ku + hu, v = asin*kv + hv
     float umax, vmax;                     // sample clamps: W (wrap) or W-1, and H-1
-    uint32_t view_bytes;                  // bytes of one packed view (4*H*(W+1))
     float var_floor, fold_agree, inv_k;
     float depth_penalty, plane_penalty;
     float dr[kMaxRefine], nr[kMaxRefine];

@@ -294,17 +294,20 @@ PMP_DF void view_group(const KfArgs& a, const float* sWc, const RefStat& rs, int
             acc2[g] = 0.0f;
         }
     }
+    // column-table and reference loads as 32-bit byte offsets from their
+    // uniform bases (offsets stay below 2^32: images hold < 2^32 pixels)
+    const char* coltab = (const char*)a.coltab;
     int t = 1;
     int k2 = -a.half + a.stride;
     for (int k1 = -a.half; k1 <= a.half; k1 += a.stride, k2 = -a.half) {
         const int ii = i + k1;
         const float2 rt = a.rowtab[ii];
-        const uint8_t* row = a.ref + (size_t)ii * W + j;
+        const uint32_t row = (uint32_t)(ii * W + j);
         for (; k2 <= a.half; k2 += a.stride, ++t) {
-            const float2 ct = a.coltab[j + k2];
+            const float2 ct = *(const float2*)(coltab + (uint32_t)(j + k2) * 8u);
             float sbx, sby, sbz, as;
             tap_dir(rt.x * ct.x, -rt.y, rt.x * ct.y, px, py, pz, sbx, sby, sbz, as);
-            const float xs = (float)row[k2] - rs.Ic;
+            const float xs = (float)a.ref[row + (uint32_t)k2] - rs.Ic;
             const float w = sWc[(int)abs_(xs)] * a.Ws[t];
             const float wxc = w * (xs - rs.mx);
 #pragma unroll
