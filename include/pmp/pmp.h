@@ -304,13 +304,18 @@ typedef struct pmp_server_opts {
 
 /* One densification request: the keyframe (pooled ids, poses, mask id, depth
  * range) plus host pixels for every id the pool does not hold yet (read only
- * during pmp_server_submit; nullable for pooled ids). */
+ * during pmp_server_submit; nullable for pooled ids), and optionally the
+ * keyframe's landmarks (world positions, 3 doubles each; copied during
+ * pmp_server_submit), which seed the maps after the random initialisation as
+ * pmp_preinit_landmarks does (PreInitSparsePCL2, depthmap.cc:361-432). */
 typedef struct pmp_request {
     pmp_keyframe kf;
     int32_t width, height;
     const uint8_t* ref_pixels;
     const uint8_t* view_pixels[PMP_MAX_VIEWS];
     const uint8_t* mask_pixels;
+    int64_t n_landmarks;
+    const double* landmark_xyz;
 } pmp_request;
 
 typedef struct pmp_request_info {

@@ -98,7 +98,8 @@ class ServerOpts(ctypes.Structure):
 
 class Request(ctypes.Structure):
     _fields_ = [("kf", Keyframe), ("width", ctypes.c_int32), ("height", ctypes.c_int32),
-                ("ref_pixels", _u8p), ("view_pixels", _u8p * PMP_MAX_VIEWS), ("mask_pixels", _u8p)]
+                ("ref_pixels", _u8p), ("view_pixels", _u8p * PMP_MAX_VIEWS), ("mask_pixels", _u8p),
+                ("n_landmarks", ctypes.c_int64), ("landmark_xyz", _f64p)]
 
 
 class RequestInfo(ctypes.Structure):
@@ -560,8 +561,9 @@ class Server:
             raise PmpError(f"{what}: {STATUS.get(rc, rc)}: {msg}")
 
     def submit(self, p: Params, ref_id, ref_pose, view_ids, view_poses, shape, ref=None, views=None, mask_id=-1,
-               mask=None, min_depth=0.0, max_depth=0.0) -> int:
-        """Queue one keyframe; arrays are needed only for ids the pool does not hold."""
+               mask=None, min_depth=0.0, max_depth=0.0, landmarks=None) -> int:
+        """Queue one keyframe; arrays are needed only for ids the pool does not hold.
+        landmarks: optional (n, 3) world positions that seed the maps after init."""
         H, W = shape
         r = Request()
         r.kf = Keyframe.make(ref_id, ref_pose, view_ids, view_poses, mask_id, min_depth, max_depth)
@@ -579,6 +581,11 @@ class Server:
         for v in range(len(view_ids)):
             r.view_pixels[v] = px(None if views is None else views[v])
         r.mask_pixels = px(mask)
+        if landmarks is not None and len(landmarks):
+            lm = np.ascontiguousarray(landmarks, dtype=np.float64).reshape(-1, 3)
+            keep.append(lm)
+            r.n_landmarks = lm.shape[0]
+            r.landmark_xyz = lm.ctypes.data_as(_f64p)
         t = ctypes.c_uint64()
         self._chk(self.L.pmp_server_submit(self.h, ctypes.byref(r), ctypes.byref(p), ctypes.byref(t)),
                   "pmp_server_submit")

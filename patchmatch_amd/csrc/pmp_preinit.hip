@@ -262,11 +262,11 @@ static int ensure_cands(pmp_ctx* ctx, PreState* s, size_t n, size_t pixels) {
     return rc;
 }
 
-// clear keys and counter, launch the candidate kernel (by the caller), apply
-static int run_stage(pmp_ctx* ctx, QueueState* q, PreState* s, const PreArgs& pa, bool zero, bool warp,
-                     int64_t* accepted) {
-    const SlotDesc PMP_CONST* sl = (const SlotDesc PMP_CONST*)q->plain.d_slots;
-    const size_t P = (size_t)ctx->W * ctx->H;
+// Clears keys and counter, launches the candidate kernel and the apply
+// kernel for the keyframe of descriptor sl (P pixels) on ctx->stream.
+static int run_stage(pmp_ctx* ctx, QueueState* q, PreState* s, const SlotDesc* sl_dev, size_t P, const PreArgs& pa,
+                     bool zero, bool warp, int64_t* accepted) {
+    const SlotDesc PMP_CONST* sl = (const SlotDesc PMP_CONST*)sl_dev;
     int rc = q_fill_u32(ctx, ctx->stream, (unsigned int*)s->keys, 2 * P, 0u);
     if (!rc) rc = q_fill_u32(ctx, ctx->stream, (unsigned int*)s->accepted, 2, 0u);
     if (rc) return rc;
@@ -300,6 +300,30 @@ static int run_stage(pmp_ctx* ctx, QueueState* q, PreState* s, const PreArgs& pa
     return PMP_OK;
 }
 
+// Landmark seeding of one keyframe slot of a descriptor set (the served
+// form): slot_dev is the slot's uploaded descriptor, whose maps hold the
+// initial planes; xyz (3 n doubles) is pinned host memory that stays valid
+// until the stream has passed this stage.  Stream-ordered on ctx->stream, no
+// host synchronisation (scratch growth aside).
+int preinit_landmarks_slot(pmp_ctx* ctx, const SlotDesc* slot_dev, int W, int H, const double* pose, int64_t n,
+                           const double* xyz, bool zero) {
+    if (n <= 0) return PMP_OK;
+    if (n > 0x7FFFFFFF) return set_error(ctx, PMP_EINVAL, "too many landmarks");
+    QueueState* q = nullptr;
+    int rc = queue_state(ctx, q);
+    PreState* s = nullptr;
+    if (!rc) rc = pre_state(ctx, s);
+    if (!rc) rc = ensure_cands(ctx, s, (size_t)n, (size_t)W * H);
+    if (!rc) rc = grow(ctx, s->xyz, s->xyz_cap, 3 * (size_t)n);
+    if (rc) return rc;
+    PMP_HIP(ctx, hipMemcpyAsync(s->xyz, xyz, 3 * (size_t)n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    PreArgs pa;
+    memcpy(pa.pose, pose, sizeof(pa.pose));
+    memset(pa.prev_pose, 0, sizeof(pa.prev_pose));
+    pa.n = (int)n;
+    return run_stage(ctx, q, s, slot_dev, (size_t)W * H, pa, zero, false, nullptr);
+}
+
 }  // namespace pmp
 
 using namespace pmp;
@@ -328,7 +352,8 @@ int pmp_preinit_landmarks(pmp_ctx* ctx, const pmp_params* params, int64_t n, con
     memcpy(pa.pose, ctx->ref_pose, sizeof(pa.pose));
     memset(pa.prev_pose, 0, sizeof(pa.prev_pose));
     pa.n = (int)n;
-    return run_stage(ctx, q, s, pa, params->edge_mode == PMP_EDGE_ZERO, false, accepted);
+    return run_stage(ctx, q, s, q->plain.d_slots, (size_t)ctx->W * ctx->H, pa, params->edge_mode == PMP_EDGE_ZERO,
+                     false, accepted);
 }
 
 int pmp_memorize_result(pmp_ctx* ctx) {
@@ -378,7 +403,7 @@ int pmp_preinit_warp(pmp_ctx* ctx, const pmp_params* params, const uint8_t* mask
     memcpy(pa.pose, ctx->ref_pose, sizeof(pa.pose));
     memcpy(pa.prev_pose, s->prev_pose, sizeof(pa.prev_pose));
     pa.n = (int)P;
-    return run_stage(ctx, q, s, pa, params->edge_mode == PMP_EDGE_ZERO, true, accepted);
+    return run_stage(ctx, q, s, q->plain.d_slots, P, pa, params->edge_mode == PMP_EDGE_ZERO, true, accepted);
 }
 
 }  // extern "C"

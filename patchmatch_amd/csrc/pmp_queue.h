@@ -89,6 +89,9 @@ struct QueueState {
 };
 
 int queue_state(pmp_ctx* ctx, QueueState*& out);
+// Landmark seeding of one slot of an uploaded descriptor set (pmp_preinit.hip).
+int preinit_landmarks_slot(pmp_ctx* ctx, const SlotDesc* slot_dev, int W, int H, const double* pose, int64_t n,
+                           const double* xyz, bool zero);
 int desc_alloc(pmp_ctx* ctx, DescSet& d, int cap);
 void desc_free(DescSet& d);
 // Waits until the staging copy of d may be rewritten (its previous upload has executed).

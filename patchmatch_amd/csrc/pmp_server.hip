@@ -5,7 +5,9 @@
 // A scheduler thread loops over steps.  A step admits queued requests into
 // free slots (keyframe maps sized from the HBM budget), then launches one
 // stage of every slot in flight, each at its own stage:
-//   stage 0                      random init + initial score   (init list)
+//   stage 0                      random init + initial score   (init list),
+//                                then landmark seeding of the requests that
+//                                carry landmarks (one launch pair each)
 //   stage 1 .. 2*iterations      red-black pass stage-1        (propagation list)
 //   stage 2*iterations + 1       copy-back, gate, PostProcess, copy to host
 // so a request that arrives while others are on the device joins at the next
@@ -61,6 +63,8 @@ struct Request {
     double t_submit = 0.0, t_admit = 0.0, t_done = 0.0;
     std::vector<uint64_t> pins;
     HostBuf out;              // plane (16 B) | depth | score | nghbr per pixel
+    HostBuf lm;               // landmark positions, 3 doubles each (pinned)
+    int64_t n_lm = 0;
 };
 
 }  // namespace
@@ -286,6 +290,7 @@ bool issue_step(pmp_server* s, int w) {
     int items_init[kMaxSlots] = {0}, items_prop[kMaxSlots] = {0}, items_rows[kMaxSlots] = {0};
     bool any = false, post = false;
     std::vector<Request*> fin;
+    std::vector<std::pair<int, Request*>> seeds;   // slots whose init runs in this step and carry landmarks
     {
         std::lock_guard<std::mutex> lk(s->mu);
         admit(s);
@@ -307,7 +312,10 @@ bool issue_step(pmp_server* s, int w) {
             ++active;
             any = true;
             r->steps++;
-            if (r->stage == 0) items_init[k] = items[0];
+            if (r->stage == 0) {
+                items_init[k] = items[0];
+                if (r->n_lm > 0) seeds.emplace_back(k, r);
+            }
             else if (r->stage < r->last_stage) items_prop[k] = items[1];
             else {
                 items_rows[k] = items[2];
@@ -329,6 +337,12 @@ bool issue_step(pmp_server* s, int w) {
     if (!rc) rc = desc_upload(ctx, d, s->max_slots, ctx->stream);
     if (!rc) rc = q_clear(ctx, ctx->stream, d, 1);
     if (!rc && d.h_hdr->init.n) rc = q_init(ctx, ctx->stream, d, run);
+    for (const auto& kr : seeds) {
+        if (rc) break;
+        const Request* r = kr.second;   // alive until fetched, which needs this step's completion
+        rc = preinit_landmarks_slot(ctx, d.d_slots + kr.first, r->W, r->H, (const double*)&r->kf.ref_pose, r->n_lm,
+                                    (const double*)r->lm.p, s->opts.edge_mode == PMP_EDGE_ZERO);
+    }
     if (!rc && d.h_hdr->prop.n) rc = q_prop(ctx, ctx->stream, d, run, 0);
     if (!rc && d.h_hdr->rows.n) rc = q_finalize(ctx, ctx->stream, d, run);
     for (Request* r : fin) {
@@ -527,6 +541,7 @@ void pmp_server_destroy(pmp_server* s) {
     (void)hipStreamSynchronize(s->ctx->stream);
     for (auto& kv : s->reqs) {
         give_host(s, kv.second->out);
+        give_host(s, kv.second->lm);
         delete kv.second;
     }
     for (auto& b : s->free_host) (void)hipHostFree(b.p);
@@ -552,6 +567,8 @@ int pmp_server_submit(pmp_server* s, const pmp_request* req, const pmp_params* p
     if (kf.n_views < 1 || kf.n_views > kMaxViews) return srv_error(s, PMP_EINVAL, "n_views must be in [1, 10]");
     if (params->edge_mode != s->opts.edge_mode) return srv_error(s, PMP_EINVAL, "edge_mode differs from the server's");
     if (params->iterations > kMaxPasses / 2) return srv_error(s, PMP_EINVAL, "iterations must be <= 64");
+    if (req->n_landmarks < 0 || req->n_landmarks > 0x7FFFFFFF || (req->n_landmarks > 0 && !req->landmark_xyz))
+        return srv_error(s, PMP_EINVAL, "n_landmarks must be in [0, 2^31) with landmark_xyz given");
     {
         std::lock_guard<std::mutex> lk(s->mu);
         if (check_params(s->ctx, params) != PMP_OK) return srv_error(s, PMP_EINVAL, s->ctx->last_error);
@@ -571,7 +588,17 @@ int pmp_server_submit(pmp_server* s, const pmp_request* req, const pmp_params* p
         rc = pool_acquire(s, kf.view_ids[v], req->view_pixels[v], W, H, wrap, true, r);
     if (rc == PMP_OK && kf.mask_id >= 0) rc = pool_acquire(s, (uint64_t)kf.mask_id, req->mask_pixels, W, H, wrap, false, r);
     std::lock_guard<std::mutex> lk(s->mu);
+    if (rc == PMP_OK && req->n_landmarks > 0) {
+        const size_t bytes = 3 * (size_t)req->n_landmarks * sizeof(double);
+        r->lm = take_host(s, bytes);
+        if (!r->lm.p) rc = srv_error(s, PMP_ENOMEM, "pinned host landmark buffer");
+        else {
+            memcpy(r->lm.p, req->landmark_xyz, bytes);
+            r->n_lm = req->n_landmarks;
+        }
+    }
     if (rc != PMP_OK) {
+        give_host(s, r->lm);
         unpin(s, r);
         delete r;
         s->cv_room.notify_all();
@@ -627,6 +654,7 @@ int pmp_server_fetch(pmp_server* s, uint64_t ticket, float* plane4, float* depth
         if (nghbr) memcpy(nghbr, o + P * 24, P * 4);
     }
     give_host(s, r->out);
+    give_host(s, r->lm);
     s->reqs.erase(it);
     delete r;
     return rc;

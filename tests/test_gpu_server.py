@@ -122,6 +122,46 @@ def test_server_inflight_pixel_budget(gpu_ctx):
         assert len(flying) == 1 or px <= budget_px, (k, flying, px)
 
 
+def test_server_landmark_seeding(gpu_ctx):
+    """A request with landmarks equals the one-keyframe sequence init ->
+    pmp_preinit_landmarks -> passes -> finalize, in the mix with a request
+    without landmarks; the seeding changes the result."""
+    kfs = _keyframes()
+    want_plain = _plain(gpu_ctx, kfs)
+    k = 0
+    kf = kfs[k]
+    p = _params(k)
+    rng = np.random.default_rng(5)
+    B = synth.bearings(kf.W, kf.H)
+    r, c = rng.integers(8, kf.H - 8, 600), rng.integers(8, kf.W - 8, 600)
+    Xc = (kf.gt_depth[r, c] * rng.uniform(0.97, 1.03, 600))[:, None] * B[r, c]
+    Xc = np.concatenate([Xc, rng.uniform(-3, 3, (40, 3))])
+    xyz = (Xc - kf.ref_pose[9:]) @ kf.ref_pose[:9].reshape(3, 3)
+    ref, views, mask = _ids(k, len(kf.views))
+    gpu_ctx.image_put(ref, kf.ref)
+    for v, img in enumerate(kf.views):
+        gpu_ctx.image_put(views[v], img)
+    gpu_ctx.set_views(ref, kf.ref_pose, views, kf.view_poses)
+    gpu_ctx.H, gpu_ctx.W = kf.H, kf.W
+    gpu_ctx.random_initialization(p)
+    acc = gpu_ctx.preinit_landmarks(p, xyz, kf.mask)
+    gpu_ctx.red_black_pass(p, 0, p.iterations)
+    gpu_ctx.finalize(p, kf.mask)
+    want = gpu_ctx.get_maps(PMP_MAPS_FINAL)
+    print(f"[server landmarks] {len(xyz)} landmarks, {acc} accepted on the plain path", flush=True)
+    assert acc > 50
+    assert not np.array_equal(want[1].view(np.uint32), want_plain[k][1].view(np.uint32))
+    with Server(0, max_slots=4) as srv:
+        srv.pause(True)
+        t0 = srv.submit(p, ref, kf.ref_pose, views, kf.view_poses, (kf.H, kf.W), ref=kf.ref, views=kf.views,
+                        mask_id=mask, mask=kf.mask, landmarks=xyz)
+        t1 = _submit(srv, kfs, 1)
+        srv.pause(False)
+        assert srv.wait(t0, 60000) and srv.wait(t1, 60000)
+        _check(srv.fetch(t0), want, "seeded request")
+        _check(srv.fetch(t1), want_plain[1], "request without landmarks beside it")
+
+
 def test_server_staggered_arrivals(gpu_ctx):
     """Requests submitted from another thread while earlier ones run."""
     kfs = _keyframes()
