@@ -27,8 +27,9 @@
  *   atan2 : q = min/max in [0,1], atan(q) = q P(q^2), octant fix-up,
  *           sign of y.
  *   lonlat: longitude atan2(x, z), latitude atan2(y, sqrt(x^2+z^2)) through
- *           the same polynomial (|y| <= h: atan(|y| rsqrt(h^2)), else
- *           pi/2 - atan(h / |y|)).
+ *           the same polynomial: rs = rsqrt(h^2 y^2) gives |y|/h = y^2 rs
+ *           and h/|y| = h^2 rs; atan of the smaller, pi/2 minus it above 45
+ *           degrees.
  *   exp   : Cody-Waite reduction x = k ln2 + r, |r| <= ln2/2, degree-6
  *           polynomial, scale by 2^k through the exponent field.
  *   log   : x = m 2^e, m in [sqrt(1/2), sqrt 2), log1p(m-1) = f P(f).
@@ -184,15 +185,17 @@ static inline float or_atan2(float y, float x) {
 }
 
 /* longitude atan2(x, z) and latitude atan2(y, h), h = sqrt(x^2 + z^2), of a
- * direction with finite components; h^2 clamped to >= 1e-24 */
+ * direction with finite components (|x|, |y|, |z| < 1e7): rs = rsqrt(h^2 y^2)
+ * gives |y|/h = y^2 rs and h/|y| = h^2 rs; the latitude is atan of the
+ * smaller (pi/2 minus it above 45 degrees).  h^2 clamped to >= 1e-24,
+ * h^2 y^2 to [1e-30, 1e30]. */
 static inline void or_lonlat(float x, float y, float z, float* lon, float* lat) {
     float h2 = or_max(fmaf(x, x, z * z), 1e-24f);
-    float ih = or_rsqrt_pos(h2);
-    float ay = fabsf(y);
-    float t = ay * ih;
-    int big = t > 1.0f;
-    if (big) t = (h2 * ih) * or_rcp_pos(or_clamp(ay, 1e-30f, 1e30f));
-    float a = or_atan_core(or_med3(t, 0.0f, 1.0f));
+    float y2 = y * y;
+    float rs = or_rsqrt_pos(or_med3(h2 * y2, 1e-30f, 1e30f));
+    float t = y2 * rs, tp = h2 * rs;
+    int big = t > tp;
+    float a = or_atan_core(or_med3(big ? tp : t, 0.0f, 1.0f));
     if (big) a = OR_PIO2_F - a;
     *lat = copysignf(a, y);
     *lon = or_atan2(x, z);

@@ -24,7 +24,7 @@ namespace pmp {
 
 // ---------------------------------------------------------------- host helpers
 int check_params(pmp_ctx* ctx, const pmp_params* p) {
-    if (p->patch_half < 3 || p->patch_stride < 1 || p->patch_half > 5 + 5)
+    if (p->patch_half < 3 || p->patch_stride < 1 || p->patch_half > kMaxHalf)
         return set_error(ctx, PMP_EINVAL, "patch_half must be in [3, 10] (propagation reads 3 pixels away)");
     if (p->n_refine < 0 || p->n_refine > kMaxRefine) return set_error(ctx, PMP_EINVAL, "n_refine must be in [0, 8]");
     if (p->agg_mode < 0 || p->agg_mode > 2 || p->edge_mode < 0 || p->edge_mode > 2 || p->coverage_mode < 0 ||

@@ -14,6 +14,7 @@ namespace pmp {
 constexpr int kMaxViews = PMP_MAX_VIEWS;
 constexpr int kMaxTaps = 121;   // patch_half 5 at stride 1
 constexpr int kMaxRefine = 8;
+constexpr int kMaxHalf = 10;   // largest patch half-width (patch_half)
 
 // Kernel argument block of one keyframe: everything uniform across the grid.
 // Passed by value (kernarg segment, read with scalar loads).

@@ -111,8 +111,17 @@ PMP_DF float ncc(const RefStat& rs, float floor_, float Swy, float Swyy, float S
 //   finite): truncation equals floor, fract(x) = x - floor(x) is exact, med3
 //   with lo <= hi is the clamp (a NaN coordinate gives lo, as clampf), and
 //   the 24-bit product covers rows and row strides below 2^24.
+// A sample in flight: the packed word being loaded and its weights.
 template <bool kZero>
-PMP_DF float sample(const KfArgs& a, const uint32_t* __restrict__ base, float u, float v) {
+struct Pend {
+    uint32_t word;
+    float fx, fy;
+    bool inside;   // kZero: the coordinate lies in [0,W-1) x [0,H-1)
+};
+
+// Issues the load of the sample at (u, v) (the blend waits for it).
+template <bool kZero>
+PMP_DF void gather(const KfArgs& a, const uint32_t* __restrict__ base, float u, float v, Pend<kZero>& p) {
     const int W = a.W;
     float uu, vv;
     bool inside = true;
@@ -124,20 +133,34 @@ PMP_DF float sample(const KfArgs& a, const uint32_t* __restrict__ base, float u,
         uu = __builtin_amdgcn_fmed3f(u, 0.0f, a.umax);
         vv = __builtin_amdgcn_fmed3f(v, 0.0f, a.vmax);
     }
-    const float fx = __builtin_amdgcn_fractf(uu), fy = __builtin_amdgcn_fractf(vv);
+    p.fx = __builtin_amdgcn_fractf(uu);
+    p.fy = __builtin_amdgcn_fractf(vv);
     const uint32_t x0 = (uint32_t)(int)uu;
     const uint32_t y0 = (uint32_t)(int)vv;
     const uint32_t idx = __umul24(y0, (uint32_t)(W + 1)) + x0;
-    const uint32_t word = *(const uint32_t*)((const char*)base + idx * 4u);
+    p.word = *(const uint32_t*)((const char*)base + idx * 4u);
+    p.inside = inside;
+}
+
+template <bool kZero>
+PMP_DF float blend(const Pend<kZero>& p) {
+    const uint32_t word = p.word;
     const float b0 = (float)(word & 0xFFu);
     const float b1 = (float)((word >> 8) & 0xFFu);
     const float b2 = (float)((word >> 16) & 0xFFu);
     const float b3 = (float)(word >> 24);
-    const float top = fma_(fx, b1 - b0, b0);
-    const float bot = fma_(fx, b3 - b2, b2);
-    const float val = fma_(fy, bot - top, top);
-    if constexpr (kZero) return inside ? val : 0.0f;
+    const float top = fma_(p.fx, b1 - b0, b0);
+    const float bot = fma_(p.fx, b3 - b2, b2);
+    const float val = fma_(p.fy, bot - top, top);
+    if constexpr (kZero) return p.inside ? val : 0.0f;
     return val;
+}
+
+template <bool kZero>
+PMP_DF float sample(const KfArgs& a, const uint32_t* __restrict__ base, float u, float v) {
+    Pend<kZero> p;
+    gather<kZero>(a, base, u, v, p);
+    return blend<kZero>(p);
 }
 
 // Equirectangular projection (PatchMatchCuda.cu:132-144): the latitude
@@ -256,15 +279,57 @@ PMP_DF void tap_dir(float bx, float by, float bz, float px, float py, float pz, 
 }
 
 template <bool kZero>
-PMP_DF float view_sample(const KfArgs& a, float hu, float hv, const uint32_t* base, int v, float sbx, float sby,
-                         float sbz, float as) {
+PMP_DF void view_gather(const KfArgs& a, float hu, float hv, const uint32_t* base, int v, float sbx, float sby,
+                        float sbz, float as, Pend<kZero>& p) {
     const float x = fma_(a.c[v][0], as, fma_(a.M[v][0], sbx, fma_(a.M[v][1], sby, a.M[v][2] * sbz)));
     const float y = fma_(a.c[v][1], as, fma_(a.M[v][3], sbx, fma_(a.M[v][4], sby, a.M[v][5] * sbz)));
     const float z = fma_(a.c[v][2], as, fma_(a.M[v][6], sbx, fma_(a.M[v][7], sby, a.M[v][8] * sbz)));
     float u, vv;
     project(a, hu, hv, x, y, z, u, vv);
-    return sample<kZero>(a, base, u, vv);
+    gather<kZero>(a, base, u, vv, p);
 }
+
+template <bool kZero>
+PMP_DF float view_sample(const KfArgs& a, float hu, float hv, const uint32_t* base, int v, float sbx, float sby,
+                         float sbz, float as) {
+    Pend<kZero> p;
+    view_gather<kZero>(a, hu, hv, base, v, sbx, sby, sbz, as, p);
+    return blend<kZero>(p);
+}
+
+// Where the per-tap inputs of the score come from.  GlobalTaps reads the
+// bearing tables and the reference image in global memory; LdsTaps reads
+// their copies for one propagation tile in LDS, so that waiting for them never
+// waits for the neighbour gathers issued before them (vector-memory loads
+// complete in order; LDS reads are counted separately).  rowoff(ii) + jj is
+// the reference pixel (ii, jj); row offsets advance by rowstep() per row.
+struct GlobalTaps {
+    const float2* rowtab;
+    const char* coltab;
+    const uint8_t* ref;
+    int W;
+    PMP_DF float2 row(int ii) const { return rowtab[ii]; }
+    PMP_DF float2 col(int jj) const { return *(const float2*)(coltab + (uint32_t)jj * 8u); }
+    PMP_DF uint32_t rowoff(int ii) const { return (uint32_t)(ii * W); }
+    PMP_DF uint32_t rowstep() const { return (uint32_t)W; }
+    PMP_DF float refpx(uint32_t roff, int jj) const { return (float)ref[roff + (uint32_t)jj]; }
+};
+PMP_DF GlobalTaps global_taps(const KfArgs& a) { return GlobalTaps{a.rowtab, (const char*)a.coltab, a.ref, a.W}; }
+
+// Tile copies: rows r0 .. r0 + kRows - 1 and columns c0 .. c0 + kCols - 1 of
+// the reference image (row stride kCols) and of the bearing tables.
+constexpr int kTileCols = 128 + 2 * kMaxHalf + 4;   // 128 tile columns and the patch halo each side
+struct LdsTaps {
+    const float2* srow;
+    const float2* scol;
+    const uint8_t* sref;
+    int r0, c0;
+    PMP_DF float2 row(int ii) const { return srow[ii - r0]; }
+    PMP_DF float2 col(int jj) const { return scol[jj - c0]; }
+    PMP_DF uint32_t rowoff(int ii) const { return (uint32_t)((ii - r0) * kTileCols - c0); }
+    PMP_DF uint32_t rowstep() const { return (uint32_t)kTileCols; }
+    PMP_DF float refpx(uint32_t roff, int jj) const { return (float)sref[roff + (uint32_t)jj]; }
+};
 
 // Moments of views [v0, v0+G) (G <= 3 live views keep the per-view
 // transforms in scalar registers; v0 is uniform).  Per view the taps are
@@ -272,18 +337,17 @@ PMP_DF float view_sample(const KfArgs& a, float hu, float hv, const uint32_t* ba
 // The first tap's sample is the view's shift, so that tap adds exact zeros
 // to the three moments (yd = 0, w >= 0): it is sampled before the loop and
 // the loop starts at the second tap.
-template <int G, bool kZero>
-PMP_DF void view_group(const KfArgs& a, const float* sWc, const RefStat& rs, int i, int j, float px, float py,
-                       float pz, int v0, float (&out)[G]) {
-    const int W = a.W;
+template <int G, bool kZero, class Taps>
+PMP_DF void view_group(const KfArgs& a, const Taps& tp, const float* sWc, const RefStat& rs, int i, int j, float px,
+                       float py, float pz, int v0, float (&out)[G]) {
     const uint32_t* base[G];
 #pragma unroll
     for (int g = 0; g < G; ++g) base[g] = a.vq[v0 + g];
     float acc0[G], acc1[G], acc2[G], shift[G];
     const float hu = in_vgpr(a.hu), hv = in_vgpr(a.hv);
     {
-        const float2 rt = a.rowtab[i - a.half];
-        const float2 ct = a.coltab[j - a.half];
+        const float2 rt = tp.row(i - a.half);
+        const float2 ct = tp.col(j - a.half);
         float sbx, sby, sbz, as;
         tap_dir(rt.x * ct.x, -rt.y, rt.x * ct.y, px, py, pz, sbx, sby, sbz, as);
 #pragma unroll
@@ -294,47 +358,84 @@ PMP_DF void view_group(const KfArgs& a, const float* sWc, const RefStat& rs, int
             acc2[g] = 0.0f;
         }
     }
-    // column-table and reference loads as 32-bit byte offsets from their
-    // uniform bases (offsets stay below 2^32: images hold < 2^32 pixels)
-    const char* coltab = (const char*)a.coltab;
-    int t = 1;
-    int k2 = -a.half + a.stride;
-    for (int k1 = -a.half; k1 <= a.half; k1 += a.stride, k2 = -a.half) {
-        const int ii = i + k1;
-        const float2 rt = a.rowtab[ii];
-        const uint32_t row = (uint32_t)(ii * W + j);
-        for (; k2 <= a.half; k2 += a.stride, ++t) {
-            const float2 ct = *(const float2*)(coltab + (uint32_t)(j + k2) * 8u);
+    // Taps t = 1 .. T-1 in order, software-pipelined: view g's gather for
+    // tap t+1 is issued right after view g's tap t is blended, into the same
+    // registers, so each load has about one tap of work to arrive.
+    const int T = a.T;
+    // tap (k1, k2) and roff = tp.rowoff(i + k1) + j, the reference offset of its row
+    auto next = [&](int& k1, int& k2, uint32_t& roff) {
+        k2 += a.stride;
+        if (k2 > a.half) {
+            k2 = -a.half;
+            k1 += a.stride;
+            roff += (uint32_t)a.stride * tp.rowstep();
+        }
+    };
+    // direction and reference weights of tap t at (k1, k2)
+    auto tap = [&](int k1, int k2, uint32_t roff, int t, float& sbx, float& sby, float& sbz, float& as, float& w,
+                   float& wxc) {
+        const float2 rt = tp.row(i + k1);
+        const float2 ct = tp.col(j + k2);
+        tap_dir(rt.x * ct.x, -rt.y, rt.x * ct.y, px, py, pz, sbx, sby, sbz, as);
+        const float xs = tp.refpx(roff, k2) - rs.Ic;
+        w = sWc[(int)abs_(xs)] * a.Ws[t];
+        wxc = w * (xs - rs.mx);
+    };
+    auto accumulate = [&](int g, float val, float w, float wxc) {
+        const float yd = val - shift[g];
+        const float tw = w * yd;
+        acc0[g] = acc0[g] + tw;
+        acc1[g] = fma_(tw, yd, acc1[g]);
+        acc2[g] = fma_(wxc, yd, acc2[g]);
+    };
+    if (T > 1) {
+        int k1 = -a.half, k2 = -a.half;
+        uint32_t roff = tp.rowoff(i - a.half) + (uint32_t)j;
+        next(k1, k2, roff);
+        Pend<kZero> pd[G];
+        float w, wxc;
+        {
             float sbx, sby, sbz, as;
-            tap_dir(rt.x * ct.x, -rt.y, rt.x * ct.y, px, py, pz, sbx, sby, sbz, as);
-            const float xs = (float)a.ref[row + (uint32_t)k2] - rs.Ic;
-            const float w = sWc[(int)abs_(xs)] * a.Ws[t];
-            const float wxc = w * (xs - rs.mx);
+            tap(k1, k2, roff, 1, sbx, sby, sbz, as, w, wxc);
+#pragma unroll
+            for (int g = 0; g < G; ++g) view_gather<kZero>(a, hu, hv, base[g], v0 + g, sbx, sby, sbz, as, pd[g]);
+        }
+        // The scheduling barriers keep each view's blend and its next gather
+        // in their own segment of the iteration; left alone the compiler
+        // gathers all loads at the end of the iteration and waits for them
+        // at the top of the next.
+#pragma unroll 1
+        for (int t = 1; t + 1 < T; ++t) {
+            next(k1, k2, roff);
+            float sbx, sby, sbz, as, nw, nwxc;
+            tap(k1, k2, roff, t + 1, sbx, sby, sbz, as, nw, nwxc);
+            __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
             for (int g = 0; g < G; ++g) {
-                const float val = view_sample<kZero>(a, hu, hv, base[g], v0 + g, sbx, sby, sbz, as);
-                const float yd = val - shift[g];
-                const float tw = w * yd;
-                acc0[g] = acc0[g] + tw;
-                acc1[g] = fma_(tw, yd, acc1[g]);
-                acc2[g] = fma_(wxc, yd, acc2[g]);
+                accumulate(g, blend<kZero>(pd[g]), w, wxc);
+                view_gather<kZero>(a, hu, hv, base[g], v0 + g, sbx, sby, sbz, as, pd[g]);
+                __builtin_amdgcn_sched_barrier(0);
             }
+            w = nw;
+            wxc = nwxc;
         }
+#pragma unroll
+        for (int g = 0; g < G; ++g) accumulate(g, blend<kZero>(pd[g]), w, wxc);
     }
 #pragma unroll
     for (int g = 0; g < G; ++g) out[g] = ncc(rs, a.var_floor, acc0[g], acc1[g], acc2[g]);
 }
 
-template <int NV, int V0, bool kZero>
-PMP_DF void view_groups(const KfArgs& a, const float* sWc, const RefStat& rs, int i, int j, float px, float py,
-                        float pz, float (&sv)[NV]) {
+template <int NV, int V0, bool kZero, class Taps>
+PMP_DF void view_groups(const KfArgs& a, const Taps& tp, const float* sWc, const RefStat& rs, int i, int j, float px,
+                        float py, float pz, float (&sv)[NV]) {
     if constexpr (V0 < NV) {
         constexpr int G = (NV - V0) < 3 ? (NV - V0) : 3;
         float out[G];
-        view_group<G, kZero>(a, sWc, rs, i, j, px, py, pz, V0, out);
+        view_group<G, kZero>(a, tp, sWc, rs, i, j, px, py, pz, V0, out);
 #pragma unroll
         for (int g = 0; g < G; ++g) sv[V0 + g] = out[g];
-        view_groups<NV, V0 + G, kZero>(a, sWc, rs, i, j, px, py, pz, sv);
+        view_groups<NV, V0 + G, kZero>(a, tp, sWc, rs, i, j, px, py, pz, sv);
     }
 }
 
@@ -342,12 +443,13 @@ PMP_DF void view_groups(const KfArgs& a, const float* sWc, const RefStat& rs, in
 // sampling.  NV > 0: compile-time view count (sv unused); NV == 0: a.N views
 // (uniform), in groups of 3, 2 or 1, with the per-view scores parked in LDS
 // at sv[v * ld] until they are aggregated.
-template <int NV, bool kZero>
-PMP_DF float score_plane(const KfArgs& a, const float* sWc, const RefStat& rs, int i, int j, float px, float py,
-                         float pz, float* sv = nullptr, int ld = 0) {
+// tp: the source of the per-tap inputs (GlobalTaps, or LdsTaps for a tile).
+template <int NV, bool kZero, class Taps>
+PMP_DF float score_plane_t(const KfArgs& a, const Taps& tp, const float* sWc, const RefStat& rs, int i, int j,
+                           float px, float py, float pz, float* sv, int ld) {
     if constexpr (NV > 0) {
         float s[NV];
-        view_groups<NV, 0, kZero>(a, sWc, rs, i, j, px, py, pz, s);
+        view_groups<NV, 0, kZero>(a, tp, sWc, rs, i, j, px, py, pz, s);
         return aggregate<NV>(a, s);
     } else {
         const int N = a.N;
@@ -355,23 +457,29 @@ PMP_DF float score_plane(const KfArgs& a, const float* sWc, const RefStat& rs, i
             const int G = (N - v0) < 3 ? (N - v0) : 3;
             if (G == 3) {
                 float out[3];
-                view_group<3, kZero>(a, sWc, rs, i, j, px, py, pz, v0, out);
+                view_group<3, kZero>(a, tp, sWc, rs, i, j, px, py, pz, v0, out);
                 sv[v0 * ld] = out[0];
                 sv[(v0 + 1) * ld] = out[1];
                 sv[(v0 + 2) * ld] = out[2];
             } else if (G == 2) {
                 float out[2];
-                view_group<2, kZero>(a, sWc, rs, i, j, px, py, pz, v0, out);
+                view_group<2, kZero>(a, tp, sWc, rs, i, j, px, py, pz, v0, out);
                 sv[v0 * ld] = out[0];
                 sv[(v0 + 1) * ld] = out[1];
             } else {
                 float out[1];
-                view_group<1, kZero>(a, sWc, rs, i, j, px, py, pz, v0, out);
+                view_group<1, kZero>(a, tp, sWc, rs, i, j, px, py, pz, v0, out);
                 sv[v0 * ld] = out[0];
             }
         }
         return aggregate_rt(a, sv, ld);
     }
+}
+
+template <int NV, bool kZero>
+PMP_DF float score_plane(const KfArgs& a, const float* sWc, const RefStat& rs, int i, int j, float px, float py,
+                         float pz, float* sv = nullptr, int ld = 0) {
+    return score_plane_t<NV, kZero>(a, global_taps(a), sWc, rs, i, j, px, py, pz, sv, ld);
 }
 
 // PlaneFromDepthNormal (PatchMatchCuda.cu:31-43)

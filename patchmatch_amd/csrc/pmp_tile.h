@@ -67,6 +67,8 @@ __device__ __forceinline__ void init_pixel(const KfArgs& a, const float* sWc, in
 // differ, so results are identical to the one-thread-per-pixel loop.
 // NVS: per-view score scratch rows (kMaxViews for the run-time view count, 1
 // otherwise).
+// srow/scol/sref: the tile's rows and columns of the bearing tables and of
+// the reference image, with the patch halo (LdsTaps).
 template <int R, int NVS = 1>
 struct PropShared {
     float sWc[256];
@@ -75,9 +77,42 @@ struct PropShared {
     float4 rs[64 * R];
     float ic[64 * R];
     float res[64 * R];
+    float2 srow[R + 2 * kMaxHalf];
+    float2 scol[kTileCols];
+    uint8_t sref[(R + 2 * kMaxHalf) * kTileCols];
     int cnt[R];
     int item;
 };
+
+// Copies the rows [tby R - half, tby R + R + half) and columns
+// [128 tbx - half, 128 tbx + 128 + half) of the reference image and of the
+// bearing tables into sh (coordinates outside the image are clamped; only
+// interior pixels, whose taps lie inside the image, read them).  Waits for the
+// previous tile's readers first.
+template <int R, int NVS>
+__device__ __forceinline__ LdsTaps stage_tile(const KfArgs& a, PropShared<R, NVS>& sh, int tbx, int tby) {
+    constexpr int NT = 64 * R;
+    const int r0 = tby * R - a.half, c0 = 128 * tbx - a.half;
+    const int nr = R + 2 * a.half, nc = 128 + 2 * a.half;
+    __syncthreads();
+    for (int e = threadIdx.x; e < nc; e += NT) {
+        const int x = min(max(c0 + e, 0), a.W - 1);
+        sh.scol[e] = a.coltab[x];
+    }
+    for (int e = threadIdx.x; e < nr; e += NT) {
+        const int y = min(max(r0 + e, 0), a.H - 1);
+        sh.srow[e] = a.rowtab[y];
+    }
+    for (int r = 0; r < nr; ++r) {
+        const int y = min(max(r0 + r, 0), a.H - 1);
+        for (int e = threadIdx.x; e < nc; e += NT) {
+            const int x = min(max(c0 + e, 0), a.W - 1);
+            sh.sref[r * kTileCols + e] = a.ref[(size_t)y * a.W + x];
+        }
+    }
+    __syncthreads();
+    return LdsTaps{sh.srow, sh.scol, sh.sref, r0, c0};
+}
 
 // One tile of one red-black pass.  sh.sWc holds a.Wc; rot (a multiple of 64)
 // is the slot offset where the packed jobs start.
@@ -95,6 +130,7 @@ __device__ __forceinline__ void propagate_tile(const KfArgs& a, PropShared<R, NV
     const int W = a.W;
     const size_t k = active ? (size_t)i * W + j : 0;
 
+    const LdsTaps tp = stage_tile(a, sh, tbx, tby);
     RefStat rs = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
     float sc = 0.0f, h0 = 0.0f, h1 = 0.0f, h2 = 0.0f, bx = 0.0f, by = 0.0f, bz = 0.0f, d0 = 1.0f, inv_d0 = 1.0f;
     if (active) {
@@ -163,7 +199,7 @@ __device__ __forceinline__ void propagate_tile(const KfArgs& a, PropShared<R, NV
             const float4 q = sh.rs[jl];
             RefStat r2;
             r2.Ic = sh.ic[jl]; r2.mx = q.x; r2.inv = q.y; r2.mxc = q.z; r2.vx = q.w;
-            sh.res[jl] = score_plane<NV, kZero>(a, sWc, r2, il, jc, jb.x, jb.y, jb.z, &sh.sv[0][tid], NT);
+            sh.res[jl] = score_plane_t<NV, kZero>(a, tp, sWc, r2, il, jc, jb.x, jb.y, jb.z, &sh.sv[0][tid], NT);
         }
         __syncthreads();
         if (need) {
@@ -191,7 +227,7 @@ __device__ __forceinline__ void propagate_tile(const KfArgs& a, PropShared<R, NV
             const float nz = fma_(a.nr[t], rc, c2);
             float q0, q1, q2;
             plane_from_depth_normal(bx, by, bz, dh, nx, ny, nz, q0, q1, q2);
-            const float sh_ = score_plane<NV, kZero>(a, sWc, rs, i, j, q0, q1, q2, &sh.sv[0][tid], NT);
+            const float sh_ = score_plane_t<NV, kZero>(a, tp, sWc, rs, i, j, q0, q1, q2, &sh.sv[0][tid], NT);
             ++n_eval;
             ++n_cand;
             if (sh_ > sc) { c0 = q0; c1 = q1; c2 = q2; sc = sh_; dcur = dh; }

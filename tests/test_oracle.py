@@ -91,7 +91,7 @@ def test_latitude_vs_libm():
     lat = or_py.eval_math("lat", a, b)
     h = np.sqrt(2.0) * np.abs(b.astype(np.float64))
     exact = np.arctan2(a.astype(np.float64), h)
-    assert ulp_err(lat, exact).max() <= 3.0
+    assert ulp_err(lat, exact).max() <= 4.0
     assert lat[-4] == np.float32(np.pi / 2) and lat[-3] == -np.float32(np.pi / 2) and lat[-2] == 0.0
 
 
