@@ -16,7 +16,8 @@ forms of the metric (patchmatch_amd/workloads.py):
   "batched": 64 keyframes of mixed resolution (960x480 / 1920x960 /
              2880x1440, weights 0.3/0.5/0.2) and view counts U{2..10}, batched
              (one work-queue launch per stage, hipGraph, autotuned) vs plain;
-  "served":  Poisson request streams at 5, 15 and 30 requests/s (resolutions
+  "served":  Poisson request streams at 5, 15 and 30 requests/s drawn from
+             a 500-keyframe trajectory (125 keyframes per resolution class,
              up to 5760x2880, views U{2..10}), served (continuous batching,
              LRU image pool) vs the plain per-request path, p50/p99 latency
              from arrival to maps on the host.
@@ -211,7 +212,8 @@ def main(argv=None):
             del st
         if args.served_requests > 0:
             t = time.perf_counter()
-            st = workloads.Stream(args.seed + 3, workloads.SERVE_CLASSES)
+            # 500 keyframes: 125 along the seeded trajectory per resolution class
+            st = workloads.Stream(args.seed + 3, workloads.SERVE_CLASSES, per_class=125, with_gt=False)
             print(f"[bench] served workload rendered in {time.perf_counter() - t:.1f}s", file=sys.stderr, flush=True)
             rates = [float(r) for r in args.rates.split(",") if r]
             result["served"] = workloads.run_served(st, p, rates, args.served_requests, args.seed + 4)

@@ -316,9 +316,12 @@ struct GlobalTaps {
 };
 PMP_DF GlobalTaps global_taps(const KfArgs& a) { return GlobalTaps{a.rowtab, (const char*)a.coltab, a.ref, a.W}; }
 
-// Tile copies: rows r0 .. r0 + kRows - 1 and columns c0 .. c0 + kCols - 1 of
-// the reference image (row stride kCols) and of the bearing tables.
-constexpr int kTileCols = 128 + 2 * kMaxHalf + 4;   // 128 tile columns and the patch halo each side
+// Tile copies: rows r0 .. r0 + R + 2 half - 1 and columns c0 .. c0 + 128 +
+// 2 half - 1 of the reference image (row stride kTileCols bytes) and of the
+// bearing tables, r0 and c0 being the tile's first row and column minus half.
+// kTileCols: 128 tile columns, the largest halo each side, padded to a
+// multiple of 8 bytes.
+constexpr int kTileCols = 128 + 2 * kMaxHalf + 4;
 struct LdsTaps {
     const float2* srow;
     const float2* scol;

@@ -288,11 +288,12 @@ def config1(seed: int = 0, W: int = 1920, H: int = 960, n_views: int = 6, worker
     return make_keyframe(seed, W, H, n_views, baseline=(0.05, 0.5), spheres=20, tube=True, workers=workers)
 
 
-def sequence_gpu(seed: int, n: int, W: int, H: int, spacing: float = 0.2, mask_frac: float = 0.15):
+def sequence_gpu(seed: int, n: int, W: int, H: int, spacing: float = 0.2, mask_frac: float = 0.15,
+                 with_gt: bool = True):
     """n keyframes along the seeded trajectory through the config-1 scene
     (room + 20 spheres + tube, spheres kept clear of the path), rendered at
     W x H on the GPU: dict with poses (n, 12), images (n x (H, W) uint8), gt
-    (n x (H, W) float32 depth) and the mask."""
+    (n x (H, W) float32 depth, or None entries without with_gt) and the mask."""
     s_scene, s_photo = _split_seeds(seed ^ 0x5E0, 2)
     scene = make_scene(s_scene, spheres=20, tube=True, cam_region=2.7)
     poses = trajectory(seed, n, spacing)
@@ -301,7 +302,7 @@ def sequence_gpu(seed: int, n: int, W: int, H: int, spacing: float = 0.2, mask_f
     for k in range(n):
         img, dep = render_gpu(scene, poses[k], W, H)
         images.append(photometric(img.astype(np.float64), prng))
-        gts.append(dep)
+        gts.append(dep if with_gt else None)
     mask = np.ones((H, W), np.uint8)
     mask[int(np.floor((1.0 - mask_frac) * H)):, :] = 0
     return {"poses": poses, "images": images, "gt": gts, "mask": mask}

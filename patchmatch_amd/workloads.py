@@ -18,6 +18,7 @@ served: a Poisson stream of requests at a given rate; the served form
 """
 from __future__ import annotations
 
+import sys
 import threading
 import time
 
@@ -30,12 +31,13 @@ SERVE_CLASSES = [(960, 480, 0.3), (1920, 960, 0.4), (2880, 1440, 0.2), (5760, 28
 class Stream:
     """Rendered trajectories per resolution class and request specs over them."""
 
-    def __init__(self, seed: int, classes, per_class: int = 16, spacing: float = 0.2):
+    def __init__(self, seed: int, classes, per_class: int = 16, spacing: float = 0.2, with_gt: bool = True):
         from . import synth
         self.classes = classes
         self.seqs = []
         for c, (W, H, _) in enumerate(classes):
-            self.seqs.append(synth.sequence_gpu(seed * 31 + c, per_class, W, H, spacing))
+            self.seqs.append(synth.sequence_gpu(seed * 31 + c, per_class, W, H, spacing, with_gt=with_gt))
+            print(f"[workloads] rendered {per_class} keyframes at {W}x{H}", file=sys.stderr, flush=True)
         self.per_class = per_class
 
     def image_id(self, c, k):
@@ -227,7 +229,9 @@ def run_served(stream: Stream, params, rates, n_requests: int, seed: int):
     for rate in rates:
         arr = _arrivals(np.random.default_rng(seed + int(rate * 1000)), rate, n_requests)
         plain = run_plain_stream(stream, params, specs, arr)
+        print(f"[workloads] {rate:g} req/s plain: {plain}", file=sys.stderr, flush=True)
         served = run_served_stream(stream, params, specs, arr)
+        print(f"[workloads] {rate:g} req/s served: {served}", file=sys.stderr, flush=True)
         out[f"{rate:g}_per_s"] = {"plain": plain, "served": served,
                                   "throughput_ratio": round(served["throughput_per_s"] / plain["throughput_per_s"], 3),
                                   "p99_ratio": round(served["p99_ms"] / plain["p99_ms"], 3)}
