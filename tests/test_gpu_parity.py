@@ -132,6 +132,28 @@ def test_propagate_c0(gpu_ctx, kf_c0):
     assert np.array_equal(np.isnan(gs), np.isnan(os_))
 
 
+@pytest.mark.parametrize("edge", [EDGE_CLAMP, EDGE_ZERO])
+def test_propagate_largest_patch(gpu_ctx, edge):
+    """The largest patch (half 10, stride 2: 121 taps, the widest halo of the
+    per-tile LDS copies) on a 200x100 keyframe whose width is not a multiple
+    of the 128-column tiles: init and one red-black iteration bit for bit."""
+    from patchmatch_amd import synth
+    kf = synth.config1(9, W=200, H=100, n_views=3)
+    p = config1_params(patch_half=10, patch_stride=2, iterations=1, edge_mode=edge, seed=31)
+    _load(gpu_ctx, kf)
+    gpu_ctx.random_initialization(p)
+    gpu_ctx.red_black_pass(p, 0, 1)
+    ok = or_py.Keyframe.from_synth(kf)
+    op, od, os_ = or_py.init(ok, p)
+    d_init = od.copy()
+    op, od, os_, _ = or_py.propagate(ok, p, op, od, os_, 0, 1)
+    gp, gd, gs = gpu_ctx.get_maps(0)
+    for name, g, o in (("plane", gp, op), ("depth", gd, od), ("score", gs, os_)):
+        eq = _report(f"half 10 edge {edge} {name}", g, o) | (np.isnan(g) & np.isnan(o))
+        assert eq.all(), f"{name}: {np.count_nonzero(~eq)} of {eq.size} differ"
+    assert np.count_nonzero(od != d_init) > 1000   # the pass updated pixels
+
+
 def test_densify_c0_final(gpu_ctx, kf_c0):
     p = config0_params()
     _load(gpu_ctx, kf_c0)
