@@ -83,8 +83,11 @@ __global__ __launch_bounds__(256) void k_init_q(const SlotDesc PMP_CONST* slots,
     }
 }
 
+// The zero-edge variant is held to 4 waves per SIMD (<= 128 VGPRs; left
+// alone it takes 132 and runs at 3 waves, 28 % slower); the wrap/clamp
+// variant fits by itself and is faster with the compiler's own allocation.
 template <bool kZero, int R>
-__global__ __launch_bounds__(64 * R) void k_propagate_q(const SlotDesc PMP_CONST* slots,
+__global__ __launch_bounds__(64 * R) __attribute__((amdgpu_waves_per_eu(kZero ? 4 : 1))) void k_propagate_q(const SlotDesc PMP_CONST* slots,
                                                         const BatchHdr PMP_CONST* hdr, int pass,
                                                         unsigned int* __restrict__ heads,
                                                         unsigned long long* __restrict__ counters, int bands) {
@@ -557,11 +560,16 @@ int pmp_batch_autotune(pmp_ctx* ctx, const pmp_params* params, int32_t n, const 
                 c.blocks_per_cu = bpc;
                 c.xcd_bands = bands;
                 if ((rc = queue_run(ctx, *q, c, zero, false, r))) break;
-                // same starting maps for every candidate: init, then time pass 0
-                if ((rc = q_clear(ctx, ctx->stream, q->batch, 1))) break;
+                // same starting maps for every candidate: init, then time the
+                // first passes (up to 3 iterations).  The first pass scores
+                // random planes whose gathers scatter; later passes are
+                // instruction-bound, so both kinds weigh in the choice.
+                const int timed = std::min(2 * params->iterations, 6);
+                if ((rc = q_clear(ctx, ctx->stream, q->batch, timed))) break;   // every timed pass's heads
                 if ((rc = q_init(ctx, ctx->stream, q->batch, r))) break;
                 hipError_t e = hipEventRecord(e0, ctx->stream);
-                if (e == hipSuccess) rc = q_prop(ctx, ctx->stream, q->batch, r, 0);
+                for (int pass = 0; pass < timed && e == hipSuccess && rc == 0; ++pass)
+                    rc = q_prop(ctx, ctx->stream, q->batch, r, pass);
                 if (e == hipSuccess && rc == 0) e = hipEventRecord(e1, ctx->stream);
                 if (e == hipSuccess && rc == 0) e = hipEventSynchronize(e1);
                 float ms = 0.0f;
