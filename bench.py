@@ -20,7 +20,11 @@ forms of the metric (patchmatch_amd/workloads.py):
              a 500-keyframe trajectory (125 keyframes per resolution class,
              up to 5760x2880, views U{2..10}), served (continuous batching,
              LRU image pool) vs the plain per-request path, p50/p99 latency
-             from arrival to maps on the host.
+             from arrival to maps on the host;
+  "pipeline": 30 keyframes of one trajectory at the plain size, densified
+             against their 4-8 nearest neighbours, cleaned (5-map window,
+             depth within 1 %, round trip within 1 px, normals within 10
+             degrees) and fused (Pruner): accuracy after each stage.
 """
 from __future__ import annotations
 
@@ -82,6 +86,8 @@ def main(argv=None):
     ap.add_argument("--batched", type=int, default=64, help="keyframes of the batched workload (0: skip)")
     ap.add_argument("--served-requests", type=int, default=40, help="requests per served rate (0: skip)")
     ap.add_argument("--rates", type=str, default="5,15,30", help="served arrival rates (requests/s)")
+    ap.add_argument("--pipeline", type=int, default=30,
+                    help="keyframes of the densify -> clean -> fuse trajectory at the plain size (0: skip)")
     args = ap.parse_args(argv)
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -201,8 +207,14 @@ def main(argv=None):
             "evals_per_s": round(eps, 1)}
     ctx.close()
 
-    if args.batched > 0 or args.served_requests > 0:
+    if args.batched > 0 or args.served_requests > 0 or args.pipeline > 0:
         from patchmatch_amd import workloads
+        if args.pipeline > 0:
+            t = time.perf_counter()
+            result["pipeline"] = workloads.run_pipeline(args.seed + 5, args.pipeline, args.width, args.height,
+                                                        params=p)
+            print(f"[bench] pipeline in {time.perf_counter() - t:.1f}s: {result['pipeline']}", file=sys.stderr,
+                  flush=True)
         if args.batched > 0:
             t = time.perf_counter()
             st = workloads.Stream(args.seed + 1, workloads.BATCH_CLASSES)

@@ -360,6 +360,21 @@ int pmp_server_pause(pmp_server* srv, int32_t paused);
  * within same_depth_thr * depth. */
 int pmp_clean(pmp_ctx* ctx, int32_t width, int32_t height, int32_t n_maps, const float* const* depth_maps,
               const pmp_pose* poses, float same_depth_thr, int32_t min_consistent, float* cleaned);
+/* pmp_clean with two optional tests that a view must also pass to count
+ * (the reference has neither; multi-view fusion filters of this kind bound
+ * the reprojection and normal disagreement):
+ *   max_reproj_px > 0: the point map o holds at the reprojection (its
+ *     interpolated depth along the bearing of that continuous pixel),
+ *     projected back into map 0, lands within max_reproj_px pixels of the
+ *     pixel being cleaned (longitude wrapped);
+ *   max_normal_deg in (0, 90): the world-frame normals of the two maps (map
+ *     0 at the pixel, map o at the pixel nearest the reprojection) are at
+ *     most max_normal_deg apart; plane4_maps (n_maps host pointers, W*H*4
+ *     floats as pmp_get_maps returns them) are then required.
+ * Values <= 0 switch a test off; with both off this is pmp_clean. */
+int pmp_clean_ex(pmp_ctx* ctx, int32_t width, int32_t height, int32_t n_maps, const float* const* depth_maps,
+                 const float* const* plane4_maps, const pmp_pose* poses, float same_depth_thr, int32_t min_consistent,
+                 float max_reproj_px, float max_normal_deg, float* cleaned);
 
 /* ---- point fusion (DepthmapPruner, depthmap.cc:1405-1696) --------------- */
 

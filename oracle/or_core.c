@@ -798,6 +798,97 @@ EXPORT int or_clean(int W, int H, int n, const float* const* depths, const doubl
     return 0;
 }
 
+/* or_clean with the two optional per-view tests of pmp_clean_ex (no
+ * counterpart in the reference): a round trip of the point map o holds at
+ * the reprojection back into map 0 within sqrt(px2) pixels, and world-frame
+ * normals within max_normal_deg.  planes: n maps of W*H*4 floats (camera
+ * frame p = n/d), needed when max_normal_deg > 0.  Thresholds <= 0: test off. */
+static int clean_round_trip(const geo_t* g, const double* pose0, const double* poseo, int i, int j, float x, float y,
+                            float dr, float iw, float ih, float px2) {
+    /* bearing at the continuous pixel (x, y): lambda = 2 pi (x/W - 1/2), phi = pi (1/2 - y/H), as turns */
+    float ul = fmaf(x, iw, 0.5f);
+    ul = ul - floorf(ul);
+    float up = fmaf(y, -0.5f * ih, 0.25f);
+    if (up < 0.0f) up = up + 1.0f;
+    float sl, cl, sp, cp;
+    or_sincos2pi(ul, &sl, &cl);
+    or_sincos2pi(up, &sp, &cp);
+    const double b0 = (double)(cp * sl), b1 = (double)(-sp), b2 = (double)(cp * cl);
+    const double dd = (double)dr;
+    const double q0 = dd * b0 - poseo[9], q1 = dd * b1 - poseo[10], q2 = dd * b2 - poseo[11];
+    double X[3];
+    X[0] = poseo[0] * q0 + poseo[3] * q1 + poseo[6] * q2;
+    X[1] = poseo[1] * q0 + poseo[4] * q1 + poseo[7] * q2;
+    X[2] = poseo[2] * q0 + poseo[5] * q1 + poseo[8] * q2;
+    float x0, y0, d0;
+    if (reproject(g, pose0, X, &x0, &y0, &d0)) return 0;
+    const float Wf = (float)g->W;
+    float dx = x0 - (float)j;
+    if (dx > 0.5f * Wf) dx = dx - Wf;
+    else if (dx < -0.5f * Wf) dx = dx + Wf;
+    const float dy = y0 - (float)i;
+    return fmaf(dx, dx, dy * dy) <= px2;
+}
+
+static int clean_normal(const geo_t* g, const double* R, const double* S, const float* plane0, const float* planeo,
+                        size_t k, float x, float y, double cos2) {
+    int ix = (int)(x + 0.5f), iy = (int)(y + 0.5f);
+    if (ix > g->W - 1) ix = g->W - 1;
+    if (iy > g->H - 1) iy = g->H - 1;
+    const float* p = plane0 + 4 * k;
+    const float* q = planeo + 4 * ((size_t)iy * g->W + ix);
+    const double p0 = p[0], p1 = p[1], p2 = p[2], q0 = q[0], q1 = q[1], q2 = q[2];
+    const double a0 = R[0] * p0 + R[3] * p1 + R[6] * p2;
+    const double a1 = R[1] * p0 + R[4] * p1 + R[7] * p2;
+    const double a2 = R[2] * p0 + R[5] * p1 + R[8] * p2;
+    const double b0 = S[0] * q0 + S[3] * q1 + S[6] * q2;
+    const double b1 = S[1] * q0 + S[4] * q1 + S[7] * q2;
+    const double b2 = S[2] * q0 + S[5] * q1 + S[8] * q2;
+    const double d = a0 * b0 + a1 * b1 + a2 * b2;
+    if (!(d > 0.0)) return 0;
+    const double na = a0 * a0 + a1 * a1 + a2 * a2, nb = b0 * b0 + b1 * b1 + b2 * b2;
+    return d * d >= cos2 * (na * nb);
+}
+
+EXPORT int or_clean_ex(int W, int H, int n, const float* const* depths, const float* const* planes,
+                       const double* poses, float same_depth_thr, int min_consistent, float max_reproj_px,
+                       float max_normal_deg, float* out) {
+    const float px2 = max_reproj_px > 0.0f ? max_reproj_px * max_reproj_px : 0.0f;
+    double cos2 = 0.0;
+    if (max_normal_deg > 0.0f) {
+        const double cs = cos((double)max_normal_deg * (M_PI / 180.0));
+        cos2 = cs * cs;
+    }
+    geo_t g; geo_init(&g, W, H);
+    const float iw = (float)(1.0 / W), ih = (float)(1.0 / H);
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int i = 0; i < H; ++i)
+        for (int j = 0; j < W; ++j) {
+            size_t k = (size_t)i * W + j;
+            float dep = depths[0][k];
+            out[k] = 0.0f;
+            if ((double)dep < 1e-6) continue;
+            double Xw[3];
+            backproject(&g, poses, i, j, dep, Xw);
+            int consistent = 1;
+            for (int o = 1; o < n; ++o) {
+                float x, y, dp;
+                if (reproject(&g, poses + 12 * o, Xw, &x, &y, &dp)) continue;
+                int iy = (int)y, ix = (int)x;
+                if (!(iy >= 0 && iy < H && ix >= 0 && ix < W)) continue;
+                float dr = lin_interp(depths[o], W, H, y, x);
+                if (!(fabsf(dr - dp) < dp * same_depth_thr)) continue;
+                if (px2 > 0.0f && !clean_round_trip(&g, poses, poses + 12 * o, i, j, x, y, dr, iw, ih, px2)) continue;
+                if (cos2 > 0.0 && !clean_normal(&g, poses, poses + 12 * o, planes[0], planes[o], k, x, y, cos2))
+                    continue;
+                consistent++;
+            }
+            if (consistent >= min_consistent) out[k] = dep;
+        }
+    geo_free(&g);
+    return 0;
+}
+
 /* DepthmapPruner::Prune (depthmap.cc:1535-1696), sequential raster order.
  * depths[0] is the head map, depths[1..n-1] the later maps, which are MODIFIED
  * (pixels zeroed) exactly as the reference does.  Emits points in raster

@@ -50,6 +50,8 @@ def lib():
         L.or_densify.argtypes = [P(OrKeyframe), P(Params), _u8p, _f32p, _f32p, _f32p, _i32p, _f32p, _f32p, _f32p]
         L.or_clean.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, P(_f32p), _f64p, ctypes.c_float,
                                ctypes.c_int, _f32p]
+        L.or_clean_ex.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, P(_f32p), P(_f32p), _f64p, ctypes.c_float,
+                                  ctypes.c_int, ctypes.c_float, ctypes.c_float, _f32p]
         L.or_prune.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, P(_f32p), _f64p, _u8p, ctypes.c_float,
                                _f32p, ctypes.c_int64, _f64p, _u8p, _i32p, _i64p]
         L.or_eval_math.argtypes = [ctypes.c_int, ctypes.c_int64, _f32p, _f32p, _f32p]
@@ -72,7 +74,7 @@ def lib():
         L.or_preinit_warp.argtypes = [P(OrKeyframe), P(Params), _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, _u8p,
                                       _f64p, _f32p, _f32p, _f32p]
         for name in ("or_init", "or_score_field", "or_score_field_f64", "or_propagate", "or_propagate_rows",
-                     "or_finalize", "or_densify", "or_clean", "or_prune", "or_eval_math", "or_refine_normals",
+                     "or_finalize", "or_densify", "or_clean", "or_clean_ex", "or_prune", "or_eval_math", "or_refine_normals",
                      "or_tables", "or_derive_poses"):
             getattr(L, name).restype = ctypes.c_int
         _lib = L
@@ -199,6 +201,24 @@ def clean(depths, poses, thr, min_consistent):
     arr = (_f32p * len(depths))(*[_p(d, _f32p) for d in depths])
     out = np.zeros((H, W), np.float32)
     _chk(lib().or_clean(W, H, len(depths), arr, _p(poses, _f64p), thr, min_consistent, _p(out, _f32p)), "clean")
+    return out
+
+
+def clean_ex(depths, planes, poses, thr, min_consistent, max_reproj_px=0.0, max_normal_deg=0.0):
+    """or_clean_ex: the Cleaner with the optional round-trip and normal tests
+    (thresholds as pmp_clean_ex: pixels and degrees, <= 0 switches off)."""
+    depths = [np.ascontiguousarray(d, dtype=np.float32) for d in depths]
+    poses = np.ascontiguousarray(poses, dtype=np.float64).reshape(-1, 12)
+    H, W = depths[0].shape
+    arr = (_f32p * len(depths))(*[_p(d, _f32p) for d in depths])
+    if planes is not None:
+        planes = [np.ascontiguousarray(q, dtype=np.float32).reshape(H, W, 4) for q in planes]
+        parr = (_f32p * len(planes))(*[_p(q, _f32p) for q in planes])
+    else:
+        parr = (_f32p * len(depths))()
+    out = np.zeros((H, W), np.float32)
+    _chk(lib().or_clean_ex(W, H, len(depths), arr, parr, _p(poses, _f64p), thr, min_consistent,
+                           float(max_reproj_px), float(max_normal_deg), _p(out, _f32p)), "clean_ex")
     return out
 
 

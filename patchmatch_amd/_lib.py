@@ -41,7 +41,7 @@ EXPORTS = [
     "pmp_create", "pmp_destroy", "pmp_last_error", "pmp_default_params", "pmp_synchronize",
     "pmp_image_put", "pmp_image_drop", "pmp_image_pool_stats", "pmp_set_views",
     "pmp_random_initialization", "pmp_score_plane_depth", "pmp_red_black_pass", "pmp_finalize",
-    "pmp_densify", "pmp_set_maps", "pmp_get_maps", "pmp_eval_counts", "pmp_clean",
+    "pmp_densify", "pmp_set_maps", "pmp_get_maps", "pmp_eval_counts", "pmp_clean", "pmp_clean_ex",
     "pmp_pruner_add_view", "pmp_pruner_prune", "pmp_pruner_pop_head", "pmp_pruner_reset",
     "pmp_pruner_size", "pmp_pruner_get_map", "pmp_debug_eval_math", "pmp_debug_refine_normals",
     "pmp_set_timing", "pmp_get_timing", "pmp_set_mask", "pmp_debug_copy",
@@ -161,6 +161,7 @@ def load():
         "pmp_get_maps": ([_vp, i32, _f32p, _f32p, _f32p, _i32p], c_int),
         "pmp_eval_counts": ([_vp, _i64p, _i64p], c_int),
         "pmp_clean": ([_vp, i32, i32, i32, P(_f32p), P(Pose), f32, i32, _f32p], c_int),
+        "pmp_clean_ex": ([_vp, i32, i32, i32, P(_f32p), P(_f32p), P(Pose), f32, i32, f32, f32, _f32p], c_int),
         "pmp_pruner_add_view": ([_vp, _f32p, P(Pose), _u8p, i32, i32], c_int),
         "pmp_pruner_prune": ([_vp, f32, _f32p, i64, _f64p, _u8p, _i32p, _i64p], c_int),
         "pmp_pruner_pop_head": ([_vp], c_int),
@@ -465,6 +466,22 @@ class Context:
         out = np.zeros((H, W), np.float32)
         self._chk(self.L.pmp_clean(self.h, W, H, len(depths), arr, ps, thr, min_consistent, _p(out, _f32p)),
                   "pmp_clean")
+        return out
+
+    def clean_ex(self, depths, planes, poses, thr, min_consistent, max_reproj_px=0.0, max_normal_deg=0.0):
+        """pmp_clean_ex: clean() with the optional round-trip (pixels) and
+        normal-angle (degrees) tests; planes (H, W, 4) are needed for the latter."""
+        depths = [np.ascontiguousarray(d, dtype=np.float32) for d in depths]
+        H, W = depths[0].shape
+        arr = (_f32p * len(depths))(*[_p(d, _f32p) for d in depths])
+        parr = None
+        if planes is not None:
+            planes = [np.ascontiguousarray(q, dtype=np.float32).reshape(H, W, 4) for q in planes]
+            parr = (_f32p * len(planes))(*[_p(q, _f32p) for q in planes])
+        ps = (Pose * len(depths))(*[Pose.from12(p) for p in poses])
+        out = np.zeros((H, W), np.float32)
+        self._chk(self.L.pmp_clean_ex(self.h, W, H, len(depths), arr, parr, ps, thr, min_consistent,
+                                      float(max_reproj_px), float(max_normal_deg), _p(out, _f32p)), "pmp_clean_ex")
         return out
 
     def pruner_add_view(self, depth, pose, bgr):

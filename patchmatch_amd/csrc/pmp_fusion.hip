@@ -41,7 +41,59 @@ struct FuseArgs {
 };
 
 // ---------------------------------------------------------------- Cleaner
-__global__ void k_clean(FuseArgs a, const float* __restrict__ maps, int min_consistent, float* __restrict__ out) {
+// Optional per-view tests on top of the reference's depth test (pmp_clean_ex;
+// the reference has neither).  Both are off in pmp_clean.
+struct CleanChecks {
+    const float* planes;  // n maps of W*H*4 floats (camera-frame p = n/d), or null
+    float px2;            // squared pixel bound of the round trip; <= 0: off
+    float iw, ih;         // 1/W, 1/H
+    double cos2;          // squared cosine of the normal-angle bound; <= 0: off
+};
+
+// Round trip: the point that map o holds at (x, y), depth dr, projected back
+// into map 0 lands within sqrt(px2) pixels of (i, j) (longitude wrapped).
+__device__ __forceinline__ bool round_trip_ok(const FuseArgs& a, const CleanChecks& c, int o, int i, int j, float x,
+                                              float y, float dr) {
+    float bx, by, bz;
+    bearing_at(x, y, c.iw, c.ih, bx, by, bz);
+    double X0, X1, X2;
+    backproject_dir(a.poses[o], (double)bx, (double)by, (double)bz, dr, X0, X1, X2);
+    float x0, y0, d0;
+    if (reproject(a.g, a.poses[0], X0, X1, X2, x0, y0, d0)) return false;
+    const float Wf = (float)a.g.W;
+    float dx = x0 - (float)j;
+    if (dx > 0.5f * Wf) dx = dx - Wf;
+    else if (dx < -0.5f * Wf) dx = dx + Wf;
+    const float dy = y0 - (float)i;
+    return fma_(dx, dx, dy * dy) <= c.px2;
+}
+
+// Normals: the world-frame planes R^T p of map 0 at (i, j) and of map o at
+// the pixel nearest (x, y) are at most acos(sqrt(cos2)) apart.
+__device__ __forceinline__ bool normal_ok(const FuseArgs& a, const CleanChecks& c, int o, size_t k, float x, float y) {
+    const int W = a.g.W, H = a.g.H;
+    const int ix = min((int)(x + 0.5f), W - 1), iy = min((int)(y + 0.5f), H - 1);
+    const size_t P = (size_t)W * H;
+    const float4 p = reinterpret_cast<const float4*>(c.planes)[k];
+    const float4 q = reinterpret_cast<const float4*>(c.planes)[(size_t)o * P + (size_t)iy * W + ix];
+    const double* R = a.poses[0];
+    const double* S = a.poses[o];
+    const double p0 = (double)p.x, p1 = (double)p.y, p2 = (double)p.z;
+    const double q0 = (double)q.x, q1 = (double)q.y, q2 = (double)q.z;
+    const double a0 = R[0] * p0 + R[3] * p1 + R[6] * p2;
+    const double a1 = R[1] * p0 + R[4] * p1 + R[7] * p2;
+    const double a2 = R[2] * p0 + R[5] * p1 + R[8] * p2;
+    const double b0 = S[0] * q0 + S[3] * q1 + S[6] * q2;
+    const double b1 = S[1] * q0 + S[4] * q1 + S[7] * q2;
+    const double b2 = S[2] * q0 + S[5] * q1 + S[8] * q2;
+    const double d = a0 * b0 + a1 * b1 + a2 * b2;
+    if (!(d > 0.0)) return false;
+    const double na = a0 * a0 + a1 * a1 + a2 * a2, nb = b0 * b0 + b1 * b1 + b2 * b2;
+    return d * d >= c.cos2 * (na * nb);
+}
+
+__global__ void k_clean(FuseArgs a, CleanChecks c, const float* __restrict__ maps, int min_consistent,
+                        float* __restrict__ out) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     const int i = blockIdx.y;
     const int W = a.g.W, H = a.g.H;
@@ -60,7 +112,10 @@ __global__ void k_clean(FuseArgs a, const float* __restrict__ maps, int min_cons
             const int iy = (int)y, ix = (int)x;
             if (!(iy >= 0 && iy < H && ix >= 0 && ix < W)) continue;
             const float dr = lin_interp(maps + (size_t)o * P, W, H, y, x);
-            if (abs_(dr - dp) < dp * a.thr) consistent++;
+            if (!(abs_(dr - dp) < dp * a.thr)) continue;
+            if (c.px2 > 0.0f && !round_trip_ok(a, c, o, i, j, x, y, dr)) continue;
+            if (c.cos2 > 0.0 && !normal_ok(a, c, o, k, x, y)) continue;
+            consistent++;
         }
         if (consistent >= min_consistent) res = dep;
     }
@@ -223,10 +278,15 @@ using namespace pmp;
 
 extern "C" {
 
-int pmp_clean(pmp_ctx* ctx, int32_t width, int32_t height, int32_t n_maps, const float* const* depth_maps,
-              const pmp_pose* poses, float same_depth_thr, int32_t min_consistent, float* cleaned) {
+static int clean_impl(pmp_ctx* ctx, int32_t width, int32_t height, int32_t n_maps, const float* const* depth_maps,
+                      const float* const* plane4_maps, const pmp_pose* poses, float same_depth_thr,
+                      int32_t min_consistent, float max_reproj_px, float max_normal_deg, float* cleaned) {
     if (!ctx || !depth_maps || !poses || !cleaned || width < 2 || height < 2) return PMP_EINVAL;
     if (n_maps < 1 || n_maps > kMaxMaps) return set_error(ctx, PMP_EINVAL, "n_maps must be in [1, 16]");
+    if (!(max_reproj_px <= 0.0f || max_reproj_px > 0.0f) || !(max_normal_deg < 90.0f))
+        return set_error(ctx, PMP_EINVAL, "max_reproj_px must not be NaN; max_normal_deg must be below 90");
+    const bool use_normals = max_normal_deg > 0.0f;
+    if (use_normals && !plane4_maps) return set_error(ctx, PMP_EINVAL, "the normal test needs the plane maps");
     FuseArgs a;
     memset(&a, 0, sizeof(a));
     int rc = geo_args(ctx, width, height, a.g);
@@ -238,12 +298,14 @@ int pmp_clean(pmp_ctx* ctx, int32_t width, int32_t height, int32_t n_maps, const
         memcpy(a.poses[o] + 9, poses[o].t, 3 * sizeof(double));
     }
     const size_t P = (size_t)width * height;
-    if (ctx->clean_cap < (n_maps + 1) * P) {
+    const size_t poff = ((n_maps + 1) * P + 3) / 4 * 4;   // planes start 16-byte aligned
+    const size_t need = use_normals ? poff + 4 * (size_t)n_maps * P : (n_maps + 1) * P;
+    if (ctx->clean_cap < need) {
         if (ctx->clean_maps) (void)hipFree(ctx->clean_maps);
         ctx->clean_maps = nullptr;
         ctx->clean_cap = 0;
-        PMP_HIP(ctx, hipMalloc(&ctx->clean_maps, (n_maps + 1) * P * sizeof(float)));
-        ctx->clean_cap = (n_maps + 1) * P;
+        PMP_HIP(ctx, hipMalloc(&ctx->clean_maps, need * sizeof(float)));
+        ctx->clean_cap = need;
     }
     for (int o = 0; o < n_maps; ++o) {
         if (!depth_maps[o]) return set_error(ctx, PMP_EINVAL, "null depth map");
@@ -251,12 +313,42 @@ int pmp_clean(pmp_ctx* ctx, int32_t width, int32_t height, int32_t n_maps, const
                                     ctx->stream));
     }
     float* dout = ctx->clean_maps + n_maps * P;
+    CleanChecks c;
+    c.planes = nullptr;
+    c.px2 = max_reproj_px > 0.0f ? max_reproj_px * max_reproj_px : 0.0f;
+    c.iw = (float)(1.0 / width);
+    c.ih = (float)(1.0 / height);
+    c.cos2 = 0.0;
+    if (use_normals) {
+        float* planes = ctx->clean_maps + poff;
+        for (int o = 0; o < n_maps; ++o) {
+            if (!plane4_maps[o]) return set_error(ctx, PMP_EINVAL, "null plane map");
+            PMP_HIP(ctx, hipMemcpyAsync(planes + 4 * o * P, plane4_maps[o], 4 * P * sizeof(float),
+                                        hipMemcpyHostToDevice, ctx->stream));
+        }
+        c.planes = planes;
+        const double cs = cos((double)max_normal_deg * (M_PI / 180.0));
+        c.cos2 = cs * cs;
+    }
     dim3 blk(256), grd((width + 255) / 256, height);
-    hipLaunchKernelGGL(k_clean, grd, blk, 0, ctx->stream, a, ctx->clean_maps, min_consistent, dout);
+    hipLaunchKernelGGL(k_clean, grd, blk, 0, ctx->stream, a, c, ctx->clean_maps, min_consistent, dout);
     PMP_HIP(ctx, hipGetLastError());
     PMP_HIP(ctx, hipMemcpyAsync(cleaned, dout, P * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
     PMP_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return PMP_OK;
+}
+
+int pmp_clean(pmp_ctx* ctx, int32_t width, int32_t height, int32_t n_maps, const float* const* depth_maps,
+              const pmp_pose* poses, float same_depth_thr, int32_t min_consistent, float* cleaned) {
+    return clean_impl(ctx, width, height, n_maps, depth_maps, nullptr, poses, same_depth_thr, min_consistent, 0.0f,
+                      0.0f, cleaned);
+}
+
+int pmp_clean_ex(pmp_ctx* ctx, int32_t width, int32_t height, int32_t n_maps, const float* const* depth_maps,
+                 const float* const* plane4_maps, const pmp_pose* poses, float same_depth_thr, int32_t min_consistent,
+                 float max_reproj_px, float max_normal_deg, float* cleaned) {
+    return clean_impl(ctx, width, height, n_maps, depth_maps, plane4_maps, poses, same_depth_thr, min_consistent,
+                      max_reproj_px, max_normal_deg, cleaned);
 }
 
 int pmp_pruner_add_view(pmp_ctx* ctx, const float* cleaned_depth, const pmp_pose* pose, const uint8_t* bgr,

@@ -14,16 +14,38 @@ struct GeoArgs {
     float ku, hu, kv, hv;
 };
 
-// Backproject_equirect (depthmap.cc:106-111): R^T (d b - t)
-__device__ __forceinline__ void backproject(const GeoArgs& g, const double* pose, int i, int j, float dep,
-                                            double& X0, double& X1, double& X2) {
-    const float2 r = g.rowtab[i], c = g.coltab[j];
-    const double bx = (double)(r.x * c.x), by = (double)(-r.y), bz = (double)(r.x * c.y);
+// R^T (d b - t) for a bearing b
+__device__ __forceinline__ void backproject_dir(const double* pose, double bx, double by, double bz, float dep,
+                                                double& X0, double& X1, double& X2) {
     const double dd = (double)dep;
     const double q0 = dd * bx - pose[9], q1 = dd * by - pose[10], q2 = dd * bz - pose[11];
     X0 = pose[0] * q0 + pose[3] * q1 + pose[6] * q2;
     X1 = pose[1] * q0 + pose[4] * q1 + pose[7] * q2;
     X2 = pose[2] * q0 + pose[5] * q1 + pose[8] * q2;
+}
+
+// Backproject_equirect (depthmap.cc:106-111): R^T (d b - t), b from the bearing table
+__device__ __forceinline__ void backproject(const GeoArgs& g, const double* pose, int i, int j, float dep,
+                                            double& X0, double& X1, double& X2) {
+    const float2 r = g.rowtab[i], c = g.coltab[j];
+    backproject_dir(pose, (double)(r.x * c.x), (double)(-r.y), (double)(r.x * c.y), dep, X0, X1, X2);
+}
+
+// Bearing at a continuous pixel position (x, y), x in [0, W), y in [0, H):
+// (cos phi sin lambda, -sin phi, cos phi cos lambda) with lambda = 2 pi (x/W - 1/2)
+// and phi = pi (1/2 - y/H), the bearing table's formula; iw = 1/W, ih = 1/H.
+// The angles go to sincos2pi as turns: x/W + 1/2 (mod 1) and 1/4 - y/(2H) (mod 1).
+__device__ __forceinline__ void bearing_at(float x, float y, float iw, float ih, float& bx, float& by, float& bz) {
+    float ul = fma_(x, iw, 0.5f);
+    ul = ul - floor_(ul);
+    float up = fma_(y, -0.5f * ih, 0.25f);
+    if (up < 0.0f) up = up + 1.0f;
+    float sl, cl, sp, cp;
+    sincos2pi(ul, &sl, &cl);
+    sincos2pi(up, &sp, &cp);
+    bx = cp * sl;
+    by = -sp;
+    bz = cp * cl;
 }
 
 // ReprojectEquirect (depthmap.cc:143-161) plus the caller's distance

@@ -161,6 +161,53 @@ def render(scene: Scene, pose: np.ndarray, W: int, H: int):
     return img.reshape(H, W), depth.reshape(H, W)
 
 
+def render_planes(scene: Scene, pose: np.ndarray, W: int, H: int):
+    """Ground-truth plane map (H, W, 4) float32 of one camera: p = n / (n . X)
+    in the camera frame (X the surface point, n its normal), so p . (d b) = 1
+    as in the densifier's plane maps; the fourth component is 0."""
+    R = pose[:9].reshape(3, 3)
+    t = pose[9:]
+    C = -R.T @ t
+    B = bearings(W, H).reshape(-1, 3)
+    D = B @ R
+    depth, obj = scene.cast(C, D)
+    P = C + depth[:, None] * D
+    n = np.zeros_like(P)
+    wall = obj < 6
+    face = np.where(wall, obj // 2, 0)
+    n[wall, face[wall]] = 1.0
+    ns = len(scene.spheres)
+    for s_, (cx, cy, cz, r) in enumerate(scene.spheres):
+        m = obj == 6 + s_
+        n[m] = (P[m] - np.array([cx, cy, cz])) / r
+    if scene.tube is not None:
+        y0, z0 = scene.tube[0], scene.tube[1]
+        m = obj == 6 + ns
+        n[m, 1] = P[m, 1] - y0
+        n[m, 2] = P[m, 2] - z0
+    nc = n @ R.T                                   # rows: R n
+    Xc = depth[:, None] * B
+    out = np.zeros((H * W, 4))
+    out[:, :3] = nc / np.einsum("ij,ij->i", nc, Xc)[:, None]
+    return out.reshape(H, W, 4).astype(np.float32)
+
+
+def tilt_planes(pl: np.ndarray, rng: np.random.Generator, frac: float, angle: float) -> np.ndarray:
+    """Plane map with a random fraction of pixels tilted by `angle` radians:
+    each chosen plane vector is rotated about a random axis perpendicular to
+    it, keeping its length (the plane through the same depth tilts)."""
+    out = pl.copy()
+    m = rng.random(pl.shape[:2]) < frac
+    v = pl[m, :3].astype(np.float64)
+    r = rng.normal(size=v.shape)
+    u = v / np.linalg.norm(v, axis=1, keepdims=True)
+    w = r - np.sum(r * u, 1, keepdims=True) * u
+    w /= np.linalg.norm(w, axis=1, keepdims=True)
+    nv = np.linalg.norm(v, axis=1, keepdims=True)
+    out[m, :3] = (nv * (np.cos(angle) * u + np.sin(angle) * w)).astype(np.float32)
+    return out
+
+
 _synth_lib = None
 
 

@@ -134,6 +134,115 @@ def run_batched(stream: Stream, params, n: int, seed: int):
     }
 
 
+def run_pipeline(seed: int = 0, n: int = 30, W: int = 1920, H: int = 960, views=(4, 8), params=None,
+                 thr: float = 0.01, max_reproj_px: float = 1.0, max_normal_deg: float = 10.0, keep: bool = False):
+    """The whole dense path over n keyframes of one trajectory through the
+    config-1 scene (0.2 m spacing), as the reference's two threads run it:
+      densify  every keyframe against its U{views} nearest trajectory
+               neighbours (one batched call);
+      clean    each map against its two predecessors and two successors
+               (DepthmapCleaner: 5-deep queue, centre map first, at least 3
+               consistent views, depth within thr);
+      fuse     the cleaned maps in order (DepthmapPruner: 5-deep queue, prune
+               the head, pop it; dense_pointcloud_module.cc:56-182).
+    The same windows are also cleaned with pmp_clean_ex's round trip within
+    max_reproj_px, normals within max_normal_deg, and both.  Accuracy
+    against the rendered ground truth after each stage; with keep, the maps
+    of every stage are returned too (for parity checks)."""
+    from . import synth
+    from ._lib import Context, Keyframe
+    from .params import config1_params
+    p = params if params is not None else config1_params(seed=seed)
+    seq = synth.sequence_gpu(seed, n, W, H, 0.2)
+    poses, imgs, gts, mask = seq["poses"], seq["images"], seq["gt"], seq["mask"]
+    rng = np.random.default_rng(seed + 0x2C)
+    ctx = Context(0)
+    t0 = time.perf_counter()
+    for k in range(n):
+        ctx.image_put(k + 1, imgs[k])
+    ctx.image_put(n + 1, mask)
+    descs = []
+    for k in range(n):
+        nv = int(rng.integers(views[0], views[1] + 1))
+        nb = sorted((m for m in range(n) if m != k), key=lambda m: (abs(m - k), m))[:nv]
+        descs.append(Keyframe.make(k + 1, poses[k], [m + 1 for m in nb], [poses[m] for m in nb], mask_id=n + 1))
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    ctx.batch_densify(p, descs)
+    maps = [ctx.batch_get_maps(k, (H, W)) for k in range(n)]       # plane4, depth, score, nghbr
+    t2 = time.perf_counter()
+    order = [2, 1, 0, 3, 4]                                          # swapCenter: centre map first
+    centres = list(range(2, n - 2))
+    windows = {c: [c - 2 + o for o in order] for c in centres}
+    cleaned = {c: ctx.clean([maps[m][1] for m in windows[c]], [poses[m] for m in windows[c]], thr, 3)
+               for c in centres}
+    t3 = time.perf_counter()
+    ctx.pruner_reset()
+    fused, pruned_maps = [], {}
+    q = 0
+    for c in centres:
+        ctx.pruner_add_view(cleaned[c], poses[c], np.repeat(imgs[c][..., None], 3, -1))
+        if ctx.pruner_size() == 5:
+            h = centres[q]
+            pr, pts, cols, kps = ctx.pruner_prune(thr, (H, W))
+            ctx.pruner_pop_head()
+            fused.append((h, pts, kps))
+            pruned_maps[h] = pr
+            q += 1
+    t4 = time.perf_counter()
+    checked = {}
+    for name, px, deg in (("round_trip", max_reproj_px, 0.0), ("normals", 0.0, max_normal_deg),
+                          ("round_trip_and_normals", max_reproj_px, max_normal_deg)):
+        checked[name] = {c: ctx.clean_ex([maps[m][1] for m in windows[c]], [maps[m][0] for m in windows[c]],
+                                         [poses[m] for m in windows[c]], thr, 3, px, deg) for c in centres}
+    t5 = time.perf_counter()
+    ctx.close()
+
+    unm = mask > 0
+
+    def summary(dmaps):
+        nv = n1 = 0
+        rels = []
+        for c, d in dmaps.items():
+            valid = (d > 0) & unm
+            rel = np.abs(d.astype(np.float64) - gts[c]) / np.maximum(gts[c], 1e-9)
+            nv += valid.sum()
+            n1 += np.sum(rel[valid] <= 0.01)
+            rels.append(rel[valid])
+        r = np.concatenate(rels) if rels else np.zeros(0)
+        return {"valid_fraction": round(float(nv / (unm.sum() * len(dmaps))), 4),
+                "absrel_le_1pct_of_valid": round(float(n1 / max(nv, 1)), 4),
+                "median_absrel": round(float(np.median(r)), 5) if r.size else None}
+
+    # fused points: distance to the head keyframe's camera against the ground truth at their pixel
+    prel, npts = [], 0
+    for h, pts, kps in fused:
+        npts += len(pts)
+        if len(pts):
+            R, t = poses[h][:9].reshape(3, 3), poses[h][9:]
+            dist = np.linalg.norm(pts @ R.T + t, axis=1)
+            gt = gts[h][kps[:, 0], kps[:, 1]]
+            prel.append(np.abs(dist - gt) / gt)
+    prel = np.concatenate(prel) if prel else np.zeros(0)
+    acc = {"densified": summary({c: maps[c][1] for c in centres}), "cleaned": summary(cleaned)}
+    acc.update({f"cleaned_{k}": summary(v) for k, v in checked.items()})
+    acc["fused_points"] = {"count": int(npts), "prunes": len(fused),
+                           "absrel_le_1pct": round(float(np.mean(prel <= 0.01)), 4) if prel.size else None,
+                           "median_absrel": round(float(np.median(prel)), 5) if prel.size else None}
+    out = {
+        "keyframes": n, "width": W, "height": H, "views": list(views), "same_depth_thr": thr,
+        "max_reproj_px": max_reproj_px, "max_normal_deg": max_normal_deg,
+        "seconds": {"upload": round(t1 - t0, 3), "densify_batched": round(t2 - t1, 3), "clean": round(t3 - t2, 3),
+                    "fuse": round(t4 - t3, 3), "clean_with_checks_x3": round(t5 - t4, 3)},
+        "keyframes_per_s": round(n / (t4 - t1), 3),
+        "accuracy": acc,
+    }
+    if keep:
+        out.update(_maps=maps, _cleaned=cleaned, _checked=checked, _windows=windows, _poses=poses, _images=imgs,
+                   _fused=fused, _pruned=pruned_maps)
+    return out
+
+
 def _arrivals(rng, rate, n):
     return np.cumsum(rng.exponential(1.0 / rate, n))
 

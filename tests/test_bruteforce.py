@@ -213,6 +213,126 @@ def test_cleaner_bruteforce_and_ground_truth():
     assert np.mean(or_py.clean(bad, traj2, 0.01, 2) > 0) < 0.05     # survivors sit on depth edges
 
 
+def py_clean_ex(depths, planes, poses, thr, min_views, px, deg):
+    """The Cleaner plus the round-trip and normal tests, float64 and libm:
+    the point map o holds at the reprojection (its bilinear depth along the
+    bearing of that continuous pixel) projected back into map 0 within px
+    pixels (longitude wrapped); world normals R^T p within deg degrees."""
+    H, W = depths[0].shape
+    B = synth.bearings(W, H)
+    R0, t0 = poses[0][:9].reshape(3, 3), poses[0][9:]
+    out = np.zeros((H, W), np.float32)
+
+    def project(R, t, X):
+        Xc = R @ X + t
+        dist = np.linalg.norm(Xc)
+        return W * (0.5 + np.arctan2(Xc[0], Xc[2]) / (2 * np.pi)), H * (0.5 + np.arcsin(Xc[1] / dist) / np.pi), dist
+
+    for i in range(H):
+        for j in range(W):
+            d = depths[0][i, j]
+            if d < 1e-6:
+                continue
+            Xw = R0.T @ (d * B[i, j] - t0)
+            n = 1
+            for o in range(1, len(depths)):
+                Ro, to = poses[o][:9].reshape(3, 3), poses[o][9:]
+                x, y, dist = project(Ro, to, Xw)
+                if dist < 1e-4 or not (0 <= int(y) < H and 0 <= int(x) < W):
+                    continue
+                if x >= W - 1 or y >= H - 1:
+                    continue
+                ix, iy = int(x), int(y)
+                dx, dy = x - ix, y - iy
+                m = depths[o]
+                dr = (1 - dy) * ((1 - dx) * m[iy, ix] + dx * m[iy, ix + 1]) + dy * ((1 - dx) * m[iy + 1, ix] + dx * m[iy + 1, ix + 1])
+                if not abs(dr - dist) < dist * thr:
+                    continue
+                if px > 0:
+                    lam, phi = 2 * np.pi * (x / W - 0.5), np.pi * (0.5 - y / H)
+                    b = np.array([np.cos(phi) * np.sin(lam), -np.sin(phi), np.cos(phi) * np.cos(lam)])
+                    x0, y0, _ = project(R0, t0, Ro.T @ (dr * b - to))
+                    ex = (x0 - j + W / 2) % W - W / 2
+                    if ex * ex + (y0 - i) ** 2 > px * px:
+                        continue
+                if deg > 0:
+                    a = R0.T @ planes[0][i, j, :3].astype(np.float64)
+                    q = Ro.T @ planes[o][min(int(y + 0.5), H - 1), min(int(x + 0.5), W - 1), :3].astype(np.float64)
+                    c = a @ q / (np.linalg.norm(a) * np.linalg.norm(q))
+                    if not c >= np.cos(np.radians(deg)):
+                        continue
+                n += 1
+            if n >= min_views:
+                out[i, j] = d
+    return out
+
+
+def _maps_planes(seed, n, W, H, noise, tilt_frac):
+    rng = np.random.default_rng(seed)
+    traj = synth.trajectory(seed, n, 0.15)
+    scene = synth.make_scene(seed + 40, spheres=3)
+    maps, planes = [], []
+    for pose in traj:
+        _, d = synth.render(scene, pose, W, H)
+        d = d.astype(np.float32) * rng.choice([1.0, 1.0, 1.0, noise], d.shape).astype(np.float32)
+        d[rng.random(d.shape) < 0.05] = 0.0
+        maps.append(d)
+        planes.append(synth.tilt_planes(synth.render_planes(scene, pose, W, H), rng, tilt_frac, np.radians(25.0)))
+    return maps, planes, traj
+
+
+def test_cleaner_checks_bruteforce():
+    maps, planes, traj = _maps_planes(3, 5, 80, 40, 1.15, 0.3)
+    plain = or_py.clean(maps, traj, 0.03, 3)
+    off = or_py.clean_ex(maps, planes, traj, 0.03, 3, 0.0, 0.0)
+    assert np.array_equal(off.view(np.uint32), plain.view(np.uint32))
+    for px, deg in ((0.5, 0.0), (0.0, 10.0), (0.5, 10.0)):
+        o = or_py.clean_ex(maps, planes, traj, 0.03, 3, px, deg)
+        p = py_clean_ex(maps, planes, traj, 0.03, 3, px, deg)
+        agree = np.mean((o > 0) == (p > 0))
+        print(f"px {px} deg {deg}: agreement {agree:.5f}, kept {np.mean(o > 0):.3f} of plain {np.mean(plain > 0):.3f}")
+        assert agree >= 0.995
+        assert not np.any((o > 0) & (plain == 0))          # the tests only remove
+        assert np.mean(o > 0) < np.mean(plain > 0)          # and they do remove here
+        assert np.array_equal(o[o > 0], maps[0][o > 0])
+
+
+def test_cleaner_checks_closed_forms():
+    """Same pose twice: the round trip returns each pixel to itself (a half-pixel
+    or sign slip in the continuous bearing would move it by >= 0.5 px), and
+    equal planes are 0 degrees apart."""
+    scene = synth.make_scene(44, spheres=3)
+    pose = synth.trajectory(5, 1, 0.15)[0]
+    W, H = 96, 48
+    _, d = synth.render(scene, pose, W, H)
+    d = d.astype(np.float32)
+    pl = synth.render_planes(scene, pose, W, H)
+    poses = np.stack([pose, pose])
+    inner = np.zeros((H, W), bool)
+    inner[1:H - 2, 1:W - 2] = True
+    k = or_py.clean_ex([d, d], [pl, pl], poses, 0.01, 2, 0.02, 1.0)
+    assert np.mean(k[inner] > 0) > 0.99
+    # a second camera 0.3 m away: ground-truth maps make the round trip land within a fraction
+    # of a pixel wherever the depth test passes; depths of the other map scaled by 1.009 still
+    # pass the 1 % depth test but move the returned point along the other camera's ray
+    traj = synth.trajectory(5, 3, 0.15)
+    d2 = synth.render(scene, traj[2], W, H)[1].astype(np.float32)
+    d0 = synth.render(scene, traj[0], W, H)[1].astype(np.float32)
+    pp = np.stack([traj[0], traj[2]])
+    depth_only = or_py.clean_ex([d0, d2], None, pp, 0.01, 2, 0.0, 0.0) > 0
+    assert np.mean(depth_only[inner]) > 0.8
+    rt = or_py.clean_ex([d0, d2], None, pp, 0.01, 2, 0.25, 0.0) > 0
+    rt_scaled = or_py.clean_ex([d0, d2 * np.float32(1.009)], None, pp, 0.01, 2, 0.25, 0.0) > 0
+    print(f"round trip keeps {np.mean(rt[depth_only]):.4f} of ground truth, {np.mean(rt_scaled[depth_only]):.4f} "
+          f"after scaling")
+    assert np.mean(rt[depth_only]) > 0.95                  # misses: bilinear depth across occlusion edges
+    assert np.mean(rt_scaled[depth_only]) < 0.9 * np.mean(rt[depth_only])
+    # planes tilted by 25 degrees: rejected at 10, kept at 40
+    tilted = synth.tilt_planes(pl, np.random.default_rng(0), 1.0, np.radians(25.0))
+    assert np.mean(or_py.clean_ex([d, d], [pl, tilted], poses, 0.01, 2, 0.0, 10.0)[inner] > 0) < 0.01
+    assert np.mean(or_py.clean_ex([d, d], [pl, tilted], poses, 0.01, 2, 0.0, 40.0)[inner] > 0) > 0.99
+
+
 def py_prune(depths, poses, thr):
     depths = [d.astype(np.float32).copy() for d in depths]
     H, W = depths[0].shape

@@ -188,6 +188,31 @@ def test_clean_bitwise(gpu_ctx, kf_c0):
     assert _same_bits(g, o).all()
 
 
+@pytest.mark.parametrize("px,deg", [(0.5, 0.0), (0.0, 10.0), (0.75, 12.5)])
+def test_clean_checks_bitwise(gpu_ctx, px, deg):
+    """pmp_clean_ex (round trip within px pixels, normals within deg degrees)
+    against the oracle on noisy maps with a third of the planes tilted 25 degrees."""
+    from patchmatch_amd import synth
+    rng = np.random.default_rng(3)
+    traj = synth.trajectory(4, 5, 0.15)
+    scene = synth.make_scene(45, spheres=3)
+    depths, planes = [], []
+    for pose in traj:
+        _, d = synth.render(scene, pose, 240, 120)
+        d = d.astype(np.float32) * rng.choice([1.0, 1.0, 1.0, 1.02], d.shape).astype(np.float32)
+        d[rng.random(d.shape) < 0.05] = 0.0
+        depths.append(d)
+        planes.append(synth.tilt_planes(synth.render_planes(scene, pose, 240, 120), rng, 0.33, np.radians(25.0)))
+    g = gpu_ctx.clean_ex(depths, planes, traj, 0.03, 3, px, deg)
+    o = or_py.clean_ex(depths, planes, traj, 0.03, 3, px, deg)
+    plain = or_py.clean(depths, traj, 0.03, 3)
+    print(f"px {px} deg {deg}: kept {np.count_nonzero(o)} of {np.count_nonzero(plain)} without the tests", flush=True)
+    assert np.count_nonzero(o) < np.count_nonzero(plain)
+    assert _same_bits(g, o).all()
+    off = gpu_ctx.clean_ex(depths, None, traj, 0.03, 3, 0.0, 0.0)
+    assert _same_bits(off, plain).all()
+
+
 def test_prune_sequence_bitwise(gpu_ctx):
     """Five queued maps, three prunes with pops: pruned maps, modified later maps
     and emitted points equal the oracle's raster-order loop exactly."""
