@@ -1,8 +1,11 @@
 """Workload driver for rocprofv3: densify the 1920x960 + 6-view keyframe
-`--steps` times, then run a streaming copy of known size (calibration of the
+`--steps` times, optionally run the densify -> clean -> fuse trajectory of
+`--pipeline` keyframes at the same size (the streaming Cleaner and Pruner
+kernels), then run a streaming copy of known size (calibration of the
 hardware byte counters: a copy of B bytes moves B bytes in and B bytes out).
 
     python -m patchmatch_amd.profile_run --steps 2 --copy-mib 256
+    python -m patchmatch_amd.profile_run --steps 1 --pipeline 10 --copy-mib 256
 """
 from __future__ import annotations
 
@@ -23,6 +26,7 @@ def main(argv=None):
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=960)
     ap.add_argument("--views", type=int, default=6)
+    ap.add_argument("--pipeline", type=int, default=0, help="keyframes of the densify -> clean -> fuse trajectory")
     args = ap.parse_args(argv)
 
     from patchmatch_amd import synth
@@ -43,6 +47,10 @@ def main(argv=None):
             ctx.synchronize()
             e, c = ctx.eval_counts()
             print(f"[profile_run] step {s}: {1e3 * (time.perf_counter() - t):.1f} ms, {e} evaluations", flush=True)
+        if args.pipeline > 0:
+            from patchmatch_amd import workloads
+            r = workloads.run_pipeline(5, args.pipeline, args.width, args.height, params=p)
+            print(f"[profile_run] pipeline of {args.pipeline} keyframes: {r['seconds']}", flush=True)
         if args.copy_mib > 0:
             ctx.debug_copy(args.copy_mib << 20, 1)
             print(f"[profile_run] copy of {args.copy_mib} MiB done", flush=True)
