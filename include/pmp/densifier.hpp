@@ -222,6 +222,10 @@ public:
     void SetSameDepthThreshold(float t) { thr_ = t; }
     void SetMinConsistentViews(int n) { min_views_ = n; }
     void SetDepthQueueSize(int n) { queue_ = n; }
+    // Optional tests of pmp_clean_ex (no counterpart in the reference; <= 0: off):
+    // round trip into the centre map within px pixels, normals within deg degrees.
+    void SetMaxReprojectionPx(float px) { max_px_ = px; }
+    void SetMaxNormalAngleDeg(float deg) { max_deg_ = deg; }
     bool IsReadyForCompute() const { return (int)results_.size() >= queue_; }
     void AddView(const DepthmapEstimatorResult& r) { results_.push_back(r); }
     // Clean (depthmap.cc:1320-1377): the centre result is cleaned against the others
@@ -240,10 +244,27 @@ public:
         out->rows = c.rows;
         out->cols = c.cols;
         out->cleaned_depth.assign((size_t)c.rows * c.cols, 0.0f);
-        check(ctx_.get(),
-              pmp_clean(ctx_.get(), c.cols, c.rows, (int32_t)maps.size(), maps.data(), poses.data(), thr_, min_views_,
-                        out->cleaned_depth.data()),
-              "pmp_clean");
+        if (max_px_ > 0.0f || max_deg_ > 0.0f) {
+            // planes back to the 4-float layout of the maps (they were floats before the widening)
+            std::vector<std::vector<float>> p4(order.size());
+            std::vector<const float*> planes;
+            for (size_t q = 0; q < order.size(); ++q) {
+                const std::vector<double>& pl = results_[order[q]].plane;
+                p4[q].assign(pl.size() / 3 * 4, 0.0f);
+                for (size_t k = 0; k < pl.size() / 3; ++k)
+                    for (int e = 0; e < 3; ++e) p4[q][4 * k + e] = (float)pl[3 * k + e];
+                planes.push_back(p4[q].data());
+            }
+            check(ctx_.get(),
+                  pmp_clean_ex(ctx_.get(), c.cols, c.rows, (int32_t)maps.size(), maps.data(), planes.data(),
+                               poses.data(), thr_, min_views_, max_px_, max_deg_, out->cleaned_depth.data()),
+                  "pmp_clean_ex");
+        } else {
+            check(ctx_.get(),
+                  pmp_clean(ctx_.get(), c.cols, c.rows, (int32_t)maps.size(), maps.data(), poses.data(), thr_,
+                            min_views_, out->cleaned_depth.data()),
+                  "pmp_clean");
+        }
         std::memcpy(out->R, c.R, sizeof(out->R));
         std::memcpy(out->t, c.t, sizeof(out->t));
     }
@@ -259,6 +280,7 @@ private:
     float thr_ = 0.01f;    // depthmap.cc:1211
     int min_views_ = 2;
     int queue_ = 1;
+    float max_px_ = 0.0f, max_deg_ = 0.0f;
 };
 
 class DepthmapPruner {

@@ -72,6 +72,8 @@ public:
         int min_consistent_views = 3;
         double parallax_gate = 0.25;
         double fallback_max_depth = 10.0;   // for a keyframe without landmarks
+        float clean_max_reproj_px = 0.0f;   // pmp_clean_ex round trip (not in the reference; 0: off)
+        float clean_max_normal_deg = 0.0f;  // pmp_clean_ex normal angle (not in the reference; 0: off)
     };
 
     DensifyFrontend(Context& ctx, const Config& cfg) : cfg_(cfg), est_(ctx), cleaner_(ctx) {
@@ -83,6 +85,8 @@ public:
         cleaner_.SetMinConsistentViews(cfg.min_consistent_views);
         cleaner_.SetDepthQueueSize(cfg.clean_queue);
         cleaner_.SetSameDepthThreshold(cfg.same_depth_threshold);
+        cleaner_.SetMaxReprojectionPx(cfg.clean_max_reproj_px);
+        cleaner_.SetMaxNormalAngleDeg(cfg.clean_max_normal_deg);
         reset_ids();
     }
 

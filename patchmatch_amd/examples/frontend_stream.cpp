@@ -4,7 +4,8 @@
 // with pre-initialisation, 5-deep Cleaner.
 //
 //   frontend_stream <input.bin> <output.bin>
-// input : int32 W, H, n, iterations; float same_depth_threshold; pmp_params
+// input : int32 W, H, n, iterations; float same_depth_threshold,
+//         clean_max_reproj_px, clean_max_normal_deg (0: off); pmp_params
 //         (raw struct, the estimator's base params); per keyframe: uint64 id,
 //         12 doubles (R row-major, t), W*H grey, W*H mask, int64 n_landmarks,
 //         n_landmarks uint64 ids, n_landmarks*3 doubles (world xyz)
@@ -26,15 +27,17 @@ int main(int argc, char** argv) {
     FILE* out = std::fopen(argv[2], "wb");
     if (!in || !out) return 2;
     int32_t hdr[4];
-    float thr;
+    float thr[3];
     pmp_params prm;
-    if (!rd(in, hdr, sizeof(hdr)) || !rd(in, &thr, 4) || !rd(in, &prm, sizeof(prm))) return 2;
+    if (!rd(in, hdr, sizeof(hdr)) || !rd(in, thr, sizeof(thr)) || !rd(in, &prm, sizeof(prm))) return 2;
     const int W = hdr[0], H = hdr[1], n = hdr[2];
     try {
         pmp::Context ctx(0);
         pmp::DensifyFrontend::Config cfg;
         cfg.iterations = hdr[3];
-        cfg.same_depth_threshold = thr;
+        cfg.same_depth_threshold = thr[0];
+        cfg.clean_max_reproj_px = thr[1];
+        cfg.clean_max_normal_deg = thr[2];
         pmp::DensifyFrontend fe(ctx, cfg);
         pmp_params& p = fe.estimator().params();
         const int iters = p.iterations, half = p.patch_half;

@@ -37,7 +37,8 @@ def _stream(W, H, n):
     return traj, imgs, ids, xyz
 
 
-def test_frontend_stream_equals_steps(gpu_ctx, tmp_path):
+@pytest.mark.parametrize("px,deg", [(0.0, 0.0), (1.0, 30.0)])
+def test_frontend_stream_equals_steps(gpu_ctx, tmp_path, px, deg):
     if not os.path.exists(EXE):
         pytest.fail("frontend_stream not built (run __graft_entry__.build())")
     W, H, it, thr = 160, 80, 2, 0.03
@@ -49,7 +50,7 @@ def test_frontend_stream_equals_steps(gpu_ctx, tmp_path):
     inp = tmp_path / "in.bin"
     with open(inp, "wb") as f:
         f.write(np.array([W, H, len(traj), it], np.int32).tobytes())
-        f.write(np.array([thr], np.float32).tobytes())
+        f.write(np.array([thr, px, deg], np.float32).tobytes())
         f.write(ctypes.string_at(ctypes.addressof(base), ctypes.sizeof(base)))
         for k, (pose, img) in enumerate(zip(traj, imgs)):
             f.write(np.array([100 + k], np.uint64).tobytes())
@@ -93,16 +94,22 @@ def test_frontend_stream_equals_steps(gpu_ctx, tmp_path):
             gpu_ctx.preinit_warp(pk, mask)
             gpu_ctx.red_black_pass(pk, 0, pk.iterations)
             gpu_ctx.finalize(pk, mask)
-            depth_results.append((ref, gpu_ctx.get_maps(PMP_MAPS_FINAL)[1]))
+            fin = gpu_ctx.get_maps(PMP_MAPS_FINAL)
+            depth_results.append((ref, fin[1], fin[0]))
             gpu_ctx.memorize_result()
             if len(depth_results) >= 5:                # a new depth map fills the 5-deep queue
                 q = depth_results[-5:]
                 order = [2, 1, 0, 3, 4]                # swapCenter of the Cleaner (depthmap.cc:1301-1309)
-                cleaned = gpu_ctx.clean([q[o][1] for o in order], [traj[q[o][0]] for o in order], thr, 3)
+                if px > 0 or deg > 0:
+                    cleaned = gpu_ctx.clean_ex([q[o][1] for o in order], [q[o][2] for o in order],
+                                               [traj[q[o][0]] for o in order], thr, 3, px, deg)
+                else:
+                    cleaned = gpu_ctx.clean([q[o][1] for o in order], [traj[q[o][0]] for o in order], thr, 3)
                 want.append((100 + q[2][0], cleaned))
     assert accepted == want_acc and accepted[4] == 0
     print(f"[frontend] accepted {accepted}, cleaned keyframes {[kid for kid, _ in results]}", flush=True)
     assert [kid for kid, _ in results] == [kid for kid, _ in want] and len(want) >= 2
     for (kid, got), (_, exp) in zip(results, want):
         assert np.array_equal(got.view(np.uint32), exp.view(np.uint32)), kid
-        assert (got > 0).mean() > 0.05
+        if deg == 0:
+            assert (got > 0).mean() > 0.05
