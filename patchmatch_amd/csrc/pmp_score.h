@@ -366,13 +366,15 @@ PMP_DF void view_group(const KfArgs& a, const Taps& tp, const float* sWc, const 
     // registers, so each load has about one tap of work to arrive.
     const int T = a.T;
     // tap (k1, k2) and roff = tp.rowoff(i + k1) + j, the reference offset of its row
+    // (k1, k2 are the same in every lane; readfirstlane keeps them, and the
+    // row step, in scalar registers instead of per-lane copies)
+    const int first_k = __builtin_amdgcn_readfirstlane(-a.half);
     auto next = [&](int& k1, int& k2, uint32_t& roff) {
-        k2 += a.stride;
-        if (k2 > a.half) {
-            k2 = -a.half;
-            k1 += a.stride;
-            roff += (uint32_t)a.stride * tp.rowstep();
-        }
+        const int k2n = k2 + a.stride;
+        const bool wrap = k2n > a.half;
+        k2 = __builtin_amdgcn_readfirstlane(wrap ? first_k : k2n);
+        k1 = __builtin_amdgcn_readfirstlane(wrap ? k1 + a.stride : k1);
+        roff += wrap ? (uint32_t)a.stride * tp.rowstep() : 0u;
     };
     // direction and reference weights of tap t at (k1, k2)
     auto tap = [&](int k1, int k2, uint32_t roff, int t, float& sbx, float& sby, float& sbz, float& as, float& w,
