@@ -154,6 +154,29 @@ def test_propagate_largest_patch(gpu_ctx, edge):
     assert np.count_nonzero(od != d_init) > 1000   # the pass updated pixels
 
 
+@pytest.mark.parametrize("n_views,agg", [(1, AGG_TOPK_MEAN), (4, AGG_REF_FOLD), (10, AGG_TOPK_MEAN), (10, AGG_MAX)])
+def test_propagate_view_counts(gpu_ctx, n_views, agg):
+    """View counts that split into groups of 3, 2 and 1 (1; 3+1; 3+3+3+1), the
+    maximum of 10 views, and a top-3 mean over a single view (k clamped to N),
+    on a 150x74 keyframe (neither dimension a multiple of a tile): init and one
+    red-black iteration bit for bit."""
+    from patchmatch_amd import synth
+    kf = synth.make_keyframe(20 + n_views, 150, 74, n_views, baseline=(0.05, 0.5), spheres=6, tube=True)
+    p = config1_params(agg_mode=agg, iterations=1, seed=7)
+    _load(gpu_ctx, kf)
+    gpu_ctx.random_initialization(p)
+    gpu_ctx.red_black_pass(p, 0, 1)
+    ok = or_py.Keyframe.from_synth(kf)
+    op, od, os_ = or_py.init(ok, p)
+    d_init = od.copy()
+    op, od, os_, _ = or_py.propagate(ok, p, op, od, os_, 0, 1)
+    gp, gd, gs = gpu_ctx.get_maps(0)
+    for name, g, o in (("plane", gp, op), ("depth", gd, od), ("score", gs, os_)):
+        eq = _report(f"{n_views} views agg {agg} {name}", g, o) | (np.isnan(g) & np.isnan(o))
+        assert eq.all(), f"{name}: {np.count_nonzero(~eq)} of {eq.size} differ"
+    assert np.count_nonzero(od != d_init) > 500
+
+
 def test_densify_c0_final(gpu_ctx, kf_c0):
     p = config0_params()
     _load(gpu_ctx, kf_c0)
@@ -211,6 +234,19 @@ def test_clean_checks_bitwise(gpu_ctx, px, deg):
     assert _same_bits(g, o).all()
     off = gpu_ctx.clean_ex(depths, None, traj, 0.03, 3, 0.0, 0.0)
     assert _same_bits(off, plain).all()
+
+
+def test_clean_checks_reject_bad_arguments(gpu_ctx):
+    from patchmatch_amd._lib import PmpError
+    d = np.ones((40, 80), np.float32)
+    pose = np.concatenate([np.eye(3).ravel(), np.zeros(3)])
+    with pytest.raises(PmpError, match="EINVAL"):
+        gpu_ctx.clean_ex([d, d], None, [pose, pose], 0.01, 2, 0.0, 10.0)       # normals without planes
+    pl = np.zeros((40, 80, 4), np.float32)
+    with pytest.raises(PmpError, match="EINVAL"):
+        gpu_ctx.clean_ex([d, d], [pl, pl], [pose, pose], 0.01, 2, 0.0, 90.0)    # no angle bound at 90 degrees
+    with pytest.raises(PmpError, match="EINVAL"):
+        gpu_ctx.clean_ex([d, d], None, [pose, pose], 0.01, 2, float("nan"), 0.0)
 
 
 def test_prune_sequence_bitwise(gpu_ctx):
