@@ -72,6 +72,56 @@ def cpu_baseline(kf, params, init_maps, target_s: float, threads: int):
     return evals / elapsed, dict(passes=passes, evals=evals, seconds=elapsed)
 
 
+def cpu_c0_end_to_end(threads: int, repeats: int = 5):
+    """Oracle densification of the 480x240 + 2-view box-room keyframe (7x7
+    patch, 4 iterations) end to end: median seconds of `repeats` runs."""
+    from oracle import or_py
+    from patchmatch_amd import synth
+    from patchmatch_amd.params import config0_params
+    or_py.set_threads(threads)
+    kf = synth.config0(0)
+    ok = or_py.Keyframe.from_synth(kf)
+    p = config0_params()
+    ts = []
+    for _ in range(repeats):
+        t = time.perf_counter()
+        or_py.densify(ok, p, kf.mask)
+        ts.append(time.perf_counter() - t)
+    return float(np.median(ts))
+
+
+def gpu_c0_end_to_end(ctx, repeats: int = 5):
+    """The same c0 densification on the GPU (images already resident):
+    median milliseconds of `repeats` runs after one warm-up."""
+    from patchmatch_amd import synth
+    from patchmatch_amd.params import config0_params
+    kf = synth.config0(0)
+    p = config0_params()
+    ctx.load_keyframe(kf.ref, kf.views, kf.ref_pose, kf.view_poses, base_id=2000)
+    ctx.image_put(2999, kf.mask)
+    ctx.set_mask(2999)
+    ctx.densify(p)
+    ctx.synchronize()
+    ts = []
+    for _ in range(repeats):
+        t = time.perf_counter()
+        ctx.densify(p)
+        ctx.synchronize()
+        ts.append(time.perf_counter() - t)
+    return float(np.median(ts)) * 1e3
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -204,7 +254,11 @@ def main(argv=None):
                        f"same keyframe from the same initial maps ({sample['evals']} hypothesis evaluations in "
                        f"{sample['seconds']:.1f} s); depth maps/s = evaluations/s / {int(evals_per_keyframe)} "
                        f"evaluations per keyframe (init + passes, as counted on the GPU)"),
-            "evals_per_s": round(eps, 1)}
+            "evals_per_s": round(eps, 1),
+            "cpu_model": cpu_model(), "nproc": os.cpu_count(),
+            "c0_end_to_end_s": round(cpu_c0_end_to_end(threads), 3),
+            "c0_gpu_end_to_end_ms": round(gpu_c0_end_to_end(ctx), 3),
+            "c0_note": "480x240 + 2 views, 7x7 patch, 4 iterations: median of 5 densifications, oracle and GPU"}
     ctx.close()
 
     if args.batched > 0 or args.served_requests > 0 or args.pipeline > 0:
