@@ -211,6 +211,12 @@ int pmp_set_maps(pmp_ctx* ctx, const float* plane4, const float* depth, const fl
  * nghbr only for PMP_MAPS_FINAL. */
 int pmp_get_maps(pmp_ctx* ctx, int32_t which, float* plane4, float* depth, float* score, int32_t* nghbr);
 
+/* Normals of a map set (the plane of DepthmapEstimatorResult, depthmap.h:71-79,
+ * as a normal map): H*W*3 floats, the unit normal -p/|p| of the plane p = n/d.
+ * Where the plane lies in front of the camera (p . b = 1/depth > 0) it faces
+ * the camera.  (0, 0, 0) where p = 0.  Computed on the device. */
+int pmp_get_normals(pmp_ctx* ctx, int32_t which, float* normal3);
+
 /* Counters of the last pmp_red_black_pass/pmp_densify: hypotheses scored
  * (evals) and scored+pre-rejected (candidates), summed over pixels. */
 int pmp_eval_counts(pmp_ctx* ctx, int64_t* evals, int64_t* candidates);

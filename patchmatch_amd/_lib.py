@@ -41,7 +41,7 @@ EXPORTS = [
     "pmp_create", "pmp_destroy", "pmp_last_error", "pmp_default_params", "pmp_synchronize",
     "pmp_image_put", "pmp_image_drop", "pmp_image_pool_stats", "pmp_set_views",
     "pmp_random_initialization", "pmp_score_plane_depth", "pmp_red_black_pass", "pmp_finalize",
-    "pmp_densify", "pmp_set_maps", "pmp_get_maps", "pmp_eval_counts", "pmp_clean", "pmp_clean_ex",
+    "pmp_densify", "pmp_set_maps", "pmp_get_maps", "pmp_get_normals", "pmp_eval_counts", "pmp_clean", "pmp_clean_ex",
     "pmp_pruner_add_view", "pmp_pruner_prune", "pmp_pruner_pop_head", "pmp_pruner_reset",
     "pmp_pruner_size", "pmp_pruner_get_map", "pmp_debug_eval_math", "pmp_debug_refine_normals",
     "pmp_set_timing", "pmp_get_timing", "pmp_set_mask", "pmp_debug_copy",
@@ -159,6 +159,7 @@ def load():
         "pmp_densify": ([_vp, P(Params), _u8p], c_int),
         "pmp_set_maps": ([_vp, _f32p, _f32p, _f32p], c_int),
         "pmp_get_maps": ([_vp, i32, _f32p, _f32p, _f32p, _i32p], c_int),
+        "pmp_get_normals": ([_vp, i32, _f32p], c_int),
         "pmp_eval_counts": ([_vp, _i64p, _i64p], c_int),
         "pmp_clean": ([_vp, i32, i32, i32, P(_f32p), P(Pose), f32, i32, _f32p], c_int),
         "pmp_clean_ex": ([_vp, i32, i32, i32, P(_f32p), P(_f32p), P(Pose), f32, i32, f32, f32, _f32p], c_int),
@@ -420,6 +421,12 @@ class Context:
         self._chk(self.L.pmp_get_maps(self.h, which, _p(plane, _f32p), _p(depth, _f32p), _p(score, _f32p),
                                       _p(nghbr, _i32p)), "pmp_get_maps")
         return (plane, depth, score, nghbr) if which == PMP_MAPS_FINAL else (plane, depth, score)
+
+    def get_normals(self, which=PMP_MAPS_FINAL):
+        """Unit camera-facing normals -p/|p| of a map set, (H, W, 3)."""
+        out = np.zeros((self.H, self.W, 3), np.float32)
+        self._chk(self.L.pmp_get_normals(self.h, which, _p(out, _f32p)), "pmp_get_normals")
+        return out
 
     def eval_counts(self):
         e, c = ctypes.c_int64(), ctypes.c_int64()

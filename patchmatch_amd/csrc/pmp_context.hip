@@ -249,7 +249,7 @@ void pmp_destroy(pmp_ctx* ctx) {
     free_slot(ctx->ws);
     for (auto& kv : ctx->images) free_image(kv.second);
     void* bufs[] = {ctx->mask, ctx->counters, ctx->thumb.src, ctx->thumb.gray, ctx->thumb.dst, ctx->thumb.taps,
-                    ctx->clean_maps, ctx->clean_poses, ctx->prune_z, ctx->prune_alive, ctx->prune_out,
+                    ctx->clean_maps, ctx->clean_poses, ctx->normals, ctx->prune_z, ctx->prune_alive, ctx->prune_out,
                     ctx->prune_pts, ctx->prune_cols, ctx->prune_kps, ctx->prune_blk, ctx->prune_poses, ctx->dbg};
     for (void* b : bufs)
         if (b) (void)hipFree(b);

@@ -18,6 +18,7 @@
 #include <string>
 
 #include "pmp_internal.h"
+#include "pmp_math.h"
 #include "pmp_queue.h"
 
 namespace pmp {
@@ -146,6 +147,29 @@ int stage_mask(pmp_ctx* ctx, const uint8_t* mask_host, const uint8_t*& dmask) {
     PMP_HIP(ctx, hipMemcpyAsync(ctx->mask, mask_host, P, hipMemcpyHostToDevice, ctx->stream));
     dmask = ctx->mask;
     return PMP_OK;
+}
+
+}  // namespace pmp
+
+namespace pmp {
+
+// Unit camera-facing normal of a plane map: n = -p / |p| (p = n_c / d with
+// p . b = 1/depth > 0 points away from the camera); (0, 0, 0) where p = 0.
+__global__ void k_normals(const float4* __restrict__ plane, float* __restrict__ out, int P) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= P) return;
+    const float4 p = plane[k];
+    const float s = fma_(p.z, p.z, fma_(p.y, p.y, p.x * p.x));
+    float nx = 0.0f, ny = 0.0f, nz = 0.0f;
+    if (s > 0.0f) {
+        const float r = rsqrt_pos(med3_(s, 1e-30f, 1e30f));
+        nx = -p.x * r;
+        ny = -p.y * r;
+        nz = -p.z * r;
+    }
+    out[3 * (size_t)k] = nx;
+    out[3 * (size_t)k + 1] = ny;
+    out[3 * (size_t)k + 2] = nz;
 }
 
 }  // namespace pmp
@@ -286,6 +310,29 @@ int pmp_get_maps(pmp_ctx* ctx, int32_t which, float* plane4, float* depth, float
         if (which != PMP_MAPS_FINAL) return set_error(ctx, PMP_EINVAL, "nghbr exists only for the final maps");
         PMP_HIP(ctx, hipMemcpyAsync(nghbr, ctx->ws.onghbr, P * 4, hipMemcpyDeviceToHost, ctx->stream));
     }
+    PMP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return PMP_OK;
+}
+
+int pmp_get_normals(pmp_ctx* ctx, int32_t which, float* normal3) {
+    if (!ctx || !normal3) return PMP_EINVAL;
+    if (!ctx->views_set) return set_error(ctx, PMP_ESTATE, "pmp_set_views has not been called");
+    const size_t P = (size_t)ctx->W * ctx->H;
+    const float4* sp;
+    if (which == PMP_MAPS_WORK) sp = (const float4*)ctx->ws.plane;
+    else if (which == PMP_MAPS_INIT) sp = (const float4*)ctx->ws.iplane;
+    else if (which == PMP_MAPS_FINAL) sp = (const float4*)ctx->ws.oplane;
+    else return set_error(ctx, PMP_EINVAL, "unknown map set");
+    if (ctx->normals_cap < 3 * P) {
+        if (ctx->normals) (void)hipFree(ctx->normals);
+        ctx->normals = nullptr;
+        ctx->normals_cap = 0;
+        PMP_HIP(ctx, hipMalloc(&ctx->normals, 3 * P * sizeof(float)));
+        ctx->normals_cap = 3 * P;
+    }
+    hipLaunchKernelGGL(k_normals, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, ctx->stream, sp, ctx->normals, (int)P);
+    PMP_HIP(ctx, hipGetLastError());
+    PMP_HIP(ctx, hipMemcpyAsync(normal3, ctx->normals, 3 * P * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
     PMP_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return PMP_OK;
 }

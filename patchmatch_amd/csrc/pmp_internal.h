@@ -112,6 +112,8 @@ struct pmp_ctx {
     // cleaner
     float* clean_maps = nullptr;
     size_t clean_cap = 0;
+    float* normals = nullptr;           // pmp_get_normals output staging, 3 floats per pixel
+    size_t normals_cap = 0;
     double* clean_poses = nullptr;
 
     // pruner

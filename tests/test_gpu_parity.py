@@ -9,6 +9,7 @@ import numpy as np
 import pytest
 
 from oracle import or_py
+from patchmatch_amd._lib import PMP_MAPS_FINAL
 from patchmatch_amd.params import (AGG_MAX, AGG_REF_FOLD, AGG_TOPK_MEAN, COVER_FULL, EDGE_CLAMP, EDGE_ZERO,
                                    config0_params, config1_params, default_params)
 
@@ -192,6 +193,29 @@ def test_densify_c0_final(gpu_ctx, kf_c0):
     _report("final score", gs, out["score"])
     _report("final plane", gp[..., :3], out["plane"])
     assert eq.mean() >= 0.999
+
+
+def test_normals_export(gpu_ctx, kf_c0):
+    """pmp_get_normals: -p/|p| of the final planes within 2 ulp of float64,
+    zero where the plane is zero; n . b has the opposite sign of p . b, so the
+    normal faces the camera wherever the plane lies in front of it."""
+    from patchmatch_amd import synth
+    p = config0_params(iterations=1)
+    _load(gpu_ctx, kf_c0)
+    gpu_ctx.densify(p, kf_c0.mask)
+    plane = gpu_ctx.get_maps(PMP_MAPS_FINAL)[0][..., :3].astype(np.float64)
+    n = gpu_ctx.get_normals(PMP_MAPS_FINAL).astype(np.float64)
+    norm = np.linalg.norm(plane, axis=-1)
+    has = norm > 0
+    want = -plane[has] / norm[has][:, None]
+    assert np.max(np.abs(n[has] - want)) <= 2.5e-7
+    assert np.all(n[~has] == 0.0)
+    B = synth.bearings(kf_c0.W, kf_c0.H)
+    pb = np.sum(plane[has] * B[has], axis=-1)
+    nb = np.sum(n[has] * B[has], axis=-1)
+    clear = np.abs(pb) > 1e-6 * norm[has]
+    assert np.all(np.sign(nb[clear]) == -np.sign(pb[clear]))
+    assert np.mean(pb > 0) > 0.99 and has.mean() > 0.5
 
 
 def test_clean_bitwise(gpu_ctx, kf_c0):
