@@ -164,27 +164,25 @@ int get_tables(pmp_ctx* ctx, int W, int H, float sigma_color, Tables& t) {
     return PMP_OK;
 }
 
-// (x,y) (x+1,y) (x,y+1) (x+1,y+1) in bytes 0..3 at word y*(W+1) + x, for
-// x in [0, W]: column x = W is column 0 again (longitude wrap: a sample at
-// u = W reads it with no index wrap) or column W-1 (clamp); x+1 of the last
-// column wraps to 0 or repeats the last column; y+1 of the last row repeats it.
-__global__ void k_pack(const uint8_t* __restrict__ g, uint32_t* __restrict__ q, int W, int H, int wrap) {
+// Column pairs: halfword (x, y) holds p(x, y) | p(x, y+1) << 8 in rows of
+// W + 2 halfwords.  Columns W and W+1 repeat columns 0 and 1 (longitude wrap:
+// a sample at u = W reads them with no index wrap) or column W-1 (clamp); y+1
+// of the last row repeats it.  A bilinear sample at (x, y), x in [0, W], is
+// the 4 bytes at halfword (x, y): p(x,y), p(x,y+1), p(x+1,y), p(x+1,y+1).
+__global__ void k_pack(const uint8_t* __restrict__ g, uint32_t* __restrict__ q32, int W, int H, int wrap) {
+    uint16_t* q = (uint16_t*)q32;
     int xo = blockIdx.x * blockDim.x + threadIdx.x;
     int y = blockIdx.y;
-    if (xo > W || y >= H) return;
-    int x = xo < W ? xo : (wrap ? 0 : W - 1);
-    int x1 = x + 1 < W ? x + 1 : (wrap ? 0 : W - 1);
+    if (xo > W + 1 || y >= H) return;
+    int x = xo < W ? xo : (wrap ? xo - W : W - 1);
     int y1 = y + 1 < H ? y + 1 : H - 1;
-    const uint8_t* r0 = g + (size_t)y * W;
-    const uint8_t* r1 = g + (size_t)y1 * W;
-    q[(size_t)y * (W + 1) + xo] = (uint32_t)r0[x] | ((uint32_t)r0[x1] << 8) | ((uint32_t)r1[x] << 16) |
-                                  ((uint32_t)r1[x1] << 24);
+    q[(size_t)y * (W + 2) + xo] = (uint16_t)(g[(size_t)y * W + x] | (g[(size_t)y1 * W + x] << 8));
 }
 
 int ensure_quads(pmp_ctx* ctx, Image& im, int wrap, hipStream_t stream) {
     if (im.quads && im.quads_wrap == wrap) return PMP_OK;
     if (!im.quads) PMP_HIP(ctx, hipMalloc(&im.quads, Image::quad_bytes(im.W, im.H)));
-    dim3 blk(256), grd((im.W + 1 + 255) / 256, im.H);
+    dim3 blk(256), grd((im.W + 2 + 255) / 256, im.H);
     hipLaunchKernelGGL(k_pack, grd, blk, 0, stream, im.gray, im.quads, im.W, im.H, wrap);
     PMP_HIP(ctx, hipGetLastError());
     im.quads_wrap = wrap;

@@ -22,7 +22,7 @@ struct KfArgs {
     int W, H, N, half, stride, T;
     int agg_mode, topk, edge_mode, coverage, init_mode, n_refine;
     const uint8_t* ref;                   // H*W
-    const uint32_t* vq[kMaxViews];        // per view: H*W packed 2x2 neighbourhood words
+    const uint32_t* vq[kMaxViews];        // per view: H*(W+2) packed column-pair halfwords (k_pack)
     const float2* rowtab;                 // H: (cos lat, sin lat)
     const float2* coltab;                 // W: (sin lon, cos lon)
     const float* Wc;                      // 256: exp(-a^2 / (2 sigma_color^2))
@@ -59,15 +59,16 @@ struct Image {
     int W = 0, H = 0;
     uint8_t* gray = nullptr;          // H*W
     size_t bytes = 0;
-    uint32_t* quads = nullptr;        // H*(W+1) words: (x,y) (x+1,y) (x,y+1) (x+1,y+1), built on first use
-    int quads_wrap = -1;              // column x+1 of the last column: 1 wraps to 0, 0 clamps; -1 not built
+    uint32_t* quads = nullptr;        // H*(W+2) column-pair halfwords p(x,y) | p(x,y+1) << 8 (k_pack), built on first use
+    int quads_wrap = -1;              // columns past the last: 1 wrap to 0, 1; 0 clamp; -1 not built
     uint64_t last_use = 0;            // pool clock of the last request that read it (LRU)
     int pins = 0;                     // requests in flight that read it (never evicted while > 0)
-    // device bytes: the grey image and, once built, its packed words
+    // device bytes: the grey image and, once built, its packed halfwords
     size_t pool_bytes() const { return bytes + (quads ? quad_bytes(W, H) : 0); }
-    // packed words of a W x H image: rows of W + 1 words, the extra column
-    // x = W repeating column 0 (wrap) or column W-1 (clamp)
-    static size_t quad_bytes(int W, int H) { return (size_t)(W + 1) * H * sizeof(uint32_t); }
+    // packed halfwords of a W x H image: rows of W + 2, the extra columns
+    // x = W, W+1 repeating columns 0, 1 (wrap) or column W-1 (clamp); rounded
+    // up to whole 32-bit words
+    static size_t quad_bytes(int W, int H) { return ((size_t)(W + 2) * H * 2 + 3) / 4 * 4; }
 };
 
 struct QueueState;   // pmp_queue.h
@@ -174,7 +175,7 @@ struct Tables {
 // Bearing tables of a W x H image and the colour-weight table of sigma_color
 // (built once per size / sigma, kept for the context's lifetime).
 int get_tables(pmp_ctx* ctx, int W, int H, float sigma_color, Tables& t);
-// Packed 2x2 words of a pooled image for the given column wrap (built on
+// Packed column pairs of a pooled image for the given column wrap (built on
 // first use, on `stream`).
 int ensure_quads(pmp_ctx* ctx, Image& im, int wrap, hipStream_t stream);
 void host_tables(int W, int H, std::vector<float2>& rowtab, std::vector<float2>& coltab);

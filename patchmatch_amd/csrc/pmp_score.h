@@ -14,7 +14,8 @@
 //   * weights Wc[|dI|] * Ws[tap] from tables;
 //   * shifted moments (reference taps centred on their weighted mean,
 //     neighbour samples shifted by the view's first-tap sample);
-//   * quad-packed uint8 neighbours: one 32-bit load per tap and view.
+//   * column-pair packed uint8 neighbours (2 bytes per texel): one 32-bit
+//     load per tap and view.
 #pragma once
 #include "pmp_internal.h"
 #include "pmp_math.h"
@@ -100,18 +101,21 @@ PMP_DF float ncc(const RefStat& rs, float floor_, float Swy, float Swyy, float S
 }
 
 // Neighbour sample at (u, v); texel k at integer k (PatchMatchCuda.cu:146 looks
-// up (u+0.5, v+0.5) with texel centres at k+0.5).  The packed words of a view
-// (rows of W + 1 words) are read with a uniform 64-bit base and a 32-bit byte
-// offset; the clamps below keep every offset inside the view (a NaN
-// coordinate clamps to 0).
+// up (u+0.5, v+0.5) with texel centres at k+0.5).  A view is packed as rows
+// of W + 2 halfwords, halfword (x, y) = p(x, y) | p(x, y+1) << 8 (k_pack); the
+// 4 bytes at halfword (x, y), a 2-byte-aligned 32-bit load with a uniform
+// 64-bit base and a 32-bit byte offset, are the sample's 2x2 neighbourhood.
+// Two bytes per texel instead of four halve the views' cache footprint.  The
+// clamps below keep every load inside the view (a NaN coordinate clamps to 0).
 //   kZero: CPU semantics (depthmap.cc:36-38), 0 outside [0,W-1) x [0,H-1);
 //   else:  u clamped to [0, umax] (umax = W with longitude wrap: column W of
-//          the words repeats column 0; W-1 for clamp), v clamped to [0, H-1].
+//          the halfwords repeats column 0, column W+1 column 1; W-1 for
+//          clamp), v clamped to [0, H-1].
 //   The instruction choices are exact for the clamped coordinates (uu, vv >= 0,
 //   finite): truncation equals floor, fract(x) = x - floor(x) is exact, med3
 //   with lo <= hi is the clamp (a NaN coordinate gives lo, as clampf), and
 //   the 24-bit product covers rows and row strides below 2^24.
-// A sample in flight: the packed word being loaded and its weights.
+// A sample in flight: the 4 packed bytes being loaded and the weights.
 template <bool kZero>
 struct Pend {
     uint32_t word;
@@ -137,20 +141,21 @@ PMP_DF void gather(const KfArgs& a, const uint32_t* __restrict__ base, float u, 
     p.fy = __builtin_amdgcn_fractf(vv);
     const uint32_t x0 = (uint32_t)(int)uu;
     const uint32_t y0 = (uint32_t)(int)vv;
-    const uint32_t idx = __umul24(y0, (uint32_t)(W + 1)) + x0;
-    p.word = *(const uint32_t*)((const char*)base + idx * 4u);
+    const uint32_t idx = __umul24(y0, (uint32_t)(W + 2)) + x0;
+    typedef uint32_t u32_a2 __attribute__((aligned(2)));
+    p.word = *(const u32_a2*)((const char*)base + idx * 2u);
     p.inside = inside;
 }
 
 template <bool kZero>
 PMP_DF float blend(const Pend<kZero>& p) {
     const uint32_t word = p.word;
-    const float b0 = (float)(word & 0xFFu);
-    const float b1 = (float)((word >> 8) & 0xFFu);
-    const float b2 = (float)((word >> 16) & 0xFFu);
-    const float b3 = (float)(word >> 24);
-    const float top = fma_(p.fx, b1 - b0, b0);
-    const float bot = fma_(p.fx, b3 - b2, b2);
+    const float b0 = (float)(word & 0xFFu);           // p(x, y)
+    const float b1 = (float)((word >> 8) & 0xFFu);    // p(x, y+1)
+    const float b2 = (float)((word >> 16) & 0xFFu);   // p(x+1, y)
+    const float b3 = (float)(word >> 24);             // p(x+1, y+1)
+    const float top = fma_(p.fx, b2 - b0, b0);
+    const float bot = fma_(p.fx, b3 - b1, b1);
     const float val = fma_(p.fy, bot - top, top);
     if constexpr (kZero) return p.inside ? val : 0.0f;
     return val;

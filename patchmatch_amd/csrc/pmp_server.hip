@@ -409,7 +409,7 @@ int reserve(pmp_server* s, std::unique_lock<std::mutex>& lk, int64_t need) {
 }
 
 // Pins the pool entry of `id` for request r, uploading it from host pixels
-// if absent and packing its 2x2 words if it serves as a neighbour (pack).
+// if absent and packing its column pairs if it serves as a neighbour (pack).
 // The entry is published (visible to the scheduler) only once its copies
 // and packing have completed.  Caller holds submit_mu, not mu.
 int pool_acquire(pmp_server* s, uint64_t id, const uint8_t* pixels, int W, int H, int wrap, bool pack, Request* r) {

@@ -192,7 +192,7 @@ def test_server_pool_lru_and_budget(gpu_ctx):
     k0 = 0   # 320x160, 6 views
     kf = kfs[k0]
     P = kf.W * kf.H
-    budget = P * 84 + P * (1 + 5 * 6 + 1) + 3 * P * 5
+    budget = P * 84 + P * (1 + 3 * 6 + 1) + 3 * P * 3     # maps; grey + 2-byte packed column pairs per view
     with Server(0, max_slots=1, hbm_budget=budget) as srv:
         t = _submit(srv, kfs, k0)
         assert srv.wait(t, 60000)
