@@ -1,9 +1,9 @@
 """GPU parity: the HIP path through the C-ABI against the CPU oracle.
 
 Integer/index work and the canonical FP32 arithmetic are compared bit for
-bit (the oracle executes the same IEEE operation sequence); the north-star
-bar of >= 99.9 % of valid pixels within 1e-4 relative depth is asserted on
-top of exact equality statistics that are printed.
+bit (the oracle executes the same IEEE operation sequence); the parity bar
+of the deterministic mode, >= 99.9 % of valid pixels within 1e-4 relative
+depth, is asserted on top of exact equality statistics that are printed.
 """
 import numpy as np
 import pytest
