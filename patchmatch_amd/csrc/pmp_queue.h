@@ -38,7 +38,7 @@ struct SlotDesc {
     int postprocess;
     int pass0;                // red-black pass of this slot at launch pass 0
     int nbx_init;             // 64-pixel tiles per row
-    int nbx_prop;             // 64 same-colour pixel tiles per row
+    int nbx_prop;             // propagation tiles per tile row (kTileSame same-colour pixels each)
 };
 
 // Items of one stage: the slots taking part and the prefix sums of their

@@ -276,9 +276,10 @@ int slot_fill(pmp_ctx* ctx, const pmp_params* params, const pmp_keyframe& kf, co
     d.postprocess = pk.postprocess;
     d.pass0 = pass0;
     d.nbx_init = (W + 63) / 64;
-    d.nbx_prop = ((W + 1) / 2 + 63) / 64;
+    d.nbx_prop = ((W + 1) / 2 + kTileSame - 1) / kTileSame;
     items[0] = d.nbx_init * ((H + 3) / 4);
-    items[1] = d.nbx_prop * ((H + rows - 1) / rows);
+    const int trows = 64 * rows / kTileSame;   // rows of a propagation tile
+    items[1] = d.nbx_prop * ((H + trows - 1) / trows);
     items[2] = H;
     return PMP_OK;
 }

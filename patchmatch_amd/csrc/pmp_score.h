@@ -321,12 +321,13 @@ struct GlobalTaps {
 };
 PMP_DF GlobalTaps global_taps(const KfArgs& a) { return GlobalTaps{a.rowtab, (const char*)a.coltab, a.ref, a.W}; }
 
-// Tile copies: rows r0 .. r0 + R + 2 half - 1 and columns c0 .. c0 + 128 +
-// 2 half - 1 of the reference image (row stride kTileCols bytes) and of the
-// bearing tables, r0 and c0 being the tile's first row and column minus half.
-// kTileCols: 128 tile columns, the largest halo each side, padded to a
+// Tile copies: the tile's rows and columns plus a halo of half on each side,
+// of the reference image (row stride kTileCols bytes) and of the bearing
+// tables, r0 and c0 being the tile's first row and column minus half.
+// kTileCols: 2 kTileSame tile columns, the largest halo each side, padded to a
 // multiple of 8 bytes.
-constexpr int kTileCols = 128 + 2 * kMaxHalf + 4;
+constexpr int kTileSame = 32;                          // same-colour pixels per tile row (a wave: 64 / kTileSame rows)
+constexpr int kTileCols = 2 * kTileSame + 2 * kMaxHalf + 4;
 struct LdsTaps {
     const float2* srow;
     const float2* scol;

@@ -77,23 +77,28 @@ struct PropShared {
     float4 rs[64 * R];
     float ic[64 * R];
     float res[64 * R];
-    float2 srow[R + 2 * kMaxHalf];
+    float2 srow[64 * R / kTileSame + 2 * kMaxHalf];
     float2 scol[kTileCols];
-    uint8_t sref[(R + 2 * kMaxHalf) * kTileCols];
+    uint8_t sref[(64 * R / kTileSame + 2 * kMaxHalf) * kTileCols];
     int cnt[R];
     int item;
 };
 
-// Copies the rows [tby R - half, tby R + R + half) and columns
-// [128 tbx - half, 128 tbx + 128 + half) of the reference image and of the
-// bearing tables into sh (coordinates outside the image are clamped; only
-// interior pixels, whose taps lie inside the image, read them).  Waits for the
-// previous tile's readers first.
+// A propagation tile is TR = 64 R / kTileSame rows by 2 kTileSame columns
+// (kTileSame same-colour pixels per row; each wave covers 64 / kTileSame
+// rows).  Taller, narrower tiles than one row per wave shrink the views' cache
+// footprint of a tile's patches.  stage_tile copies the rows
+// [tby TR - half, tby TR + TR + half) and columns [2 kTileSame tbx - half,
+// 2 kTileSame (tbx + 1) + half) of the reference image and of the bearing
+// tables into sh (coordinates outside the image are clamped; only interior
+// pixels, whose taps lie inside the image, read them).  Waits for the previous
+// tile's readers first.
 template <int R, int NVS>
 __device__ __forceinline__ LdsTaps stage_tile(const KfArgs& a, PropShared<R, NVS>& sh, int tbx, int tby) {
     constexpr int NT = 64 * R;
-    const int r0 = tby * R - a.half, c0 = 128 * tbx - a.half;
-    const int nr = R + 2 * a.half, nc = 128 + 2 * a.half;
+    constexpr int TR = NT / kTileSame;   // tile rows
+    const int r0 = tby * TR - a.half, c0 = 2 * kTileSame * tbx - a.half;
+    const int nr = TR + 2 * a.half, nc = 2 * kTileSame + 2 * a.half;
     __syncthreads();
     for (int e = threadIdx.x; e < nc; e += NT) {
         const int x = min(max(c0 + e, 0), a.W - 1);
@@ -124,8 +129,9 @@ __device__ __forceinline__ void propagate_tile(const KfArgs& a, PropShared<R, NV
     constexpr int NT = 64 * R;
     const float* sWc = sh.sWc;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int i = tby * R + wave;
-    const int j = 2 * (tbx * 64 + lane) + ((i + colour) & 1);
+    constexpr int TR = NT / kTileSame;   // tile rows
+    const int i = tby * TR + tid / kTileSame;
+    const int j = 2 * (tbx * kTileSame + tid % kTileSame) + ((i + colour) & 1);
     const bool active = i < a.H && j < a.W && in_prop_grid(a, i, j);
     const int W = a.W;
     const size_t k = active ? (size_t)i * W + j : 0;
@@ -194,8 +200,8 @@ __device__ __forceinline__ void propagate_tile(const KfArgs& a, PropShared<R, NV
         if (((tid - rot) & (NT - 1)) < total) {
             const float4 jb = sh.job[tid];
             const int jl = __float_as_int(jb.w);
-            const int il = tby * R + (jl >> 6);
-            const int jc = 2 * (tbx * 64 + (jl & 63)) + ((il + colour) & 1);
+            const int il = tby * TR + jl / kTileSame;
+            const int jc = 2 * (tbx * kTileSame + jl % kTileSame) + ((il + colour) & 1);
             const float4 q = sh.rs[jl];
             RefStat r2;
             r2.Ic = sh.ic[jl]; r2.mx = q.x; r2.inv = q.y; r2.mxc = q.z; r2.vx = q.w;
