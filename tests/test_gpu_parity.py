@@ -178,6 +178,33 @@ def test_propagate_view_counts(gpu_ctx, n_views, agg):
     assert np.count_nonzero(od != d_init) > 500
 
 
+def test_propagate_full_size_row_band(gpu_ctx):
+    """The 1920x960 keyframe with 6 neighbours at full size: the GPU's first
+    red-black iteration from its own initial maps, against the oracle
+    recomputing a band of rows from the same initial maps (red pass on the band
+    plus the 3-row reach of the candidates, then the black pass on the band).
+    Rows of one colour only read the other colour within a pass, so the band
+    equals the whole pass there: bit for bit."""
+    from patchmatch_amd import synth
+    kf = synth.config1(0, W=1920, H=960, n_views=6)
+    p = config1_params(seed=0)
+    _load(gpu_ctx, kf)
+    gpu_ctx.random_initialization(p)
+    ip, idp, isc = gpu_ctx.get_maps(0)
+    gpu_ctx.red_black_pass(p, 0, 1)
+    gp, gd, gs = gpu_ctx.get_maps(0)
+    ok = or_py.Keyframe.from_synth(kf)
+    op, od, os_ = (np.array(a, copy=True) for a in (ip, idp, isc))
+    r0, r1 = 470, 506
+    or_py.propagate_rows(ok, p, op, od, os_, 0, 0, r0 - 3, r1 + 3)
+    or_py.propagate_rows(ok, p, op, od, os_, 0, 1, r0, r1)
+    band = slice(r0, r1)
+    for name, g, o in (("plane", gp[band], op[band]), ("depth", gd[band], od[band]), ("score", gs[band], os_[band])):
+        eq = _report(f"1920x960 rows {r0}-{r1} {name}", g, o) | (np.isnan(g) & np.isnan(o))
+        assert eq.all(), f"{name}: {np.count_nonzero(~eq)} of {eq.size} differ"
+    assert np.count_nonzero(od[band] != idp[band]) > 0.3 * od[band].size
+
+
 def test_densify_c0_final(gpu_ctx, kf_c0):
     p = config0_params()
     _load(gpu_ctx, kf_c0)
