@@ -205,6 +205,35 @@ def test_propagate_full_size_row_band(gpu_ctx):
     assert np.count_nonzero(od[band] != idp[band]) > 0.3 * od[band].size
 
 
+def test_propagate_largest_served_size_row_band(gpu_ctx):
+    """The largest keyframes of the served stream, 5760x2880 with 4 neighbours
+    from one trajectory (rendered on the GPU): the first iteration on a
+    12-row band recomputed by the oracle, bit for bit.  Covers the 24-bit
+    index products and byte offsets at the largest width and height."""
+    from patchmatch_amd import synth
+    seq = synth.sequence_gpu(17, 5, 5760, 2880, with_gt=False)
+    ims, poses = seq["images"], seq["poses"]
+    kf = synth.Keyframe(W=5760, H=2880, ref=ims[2], views=[ims[k] for k in (0, 1, 3, 4)],
+                        ref_pose=poses[2], view_poses=np.stack([poses[k] for k in (0, 1, 3, 4)]),
+                        gt_depth=None, mask=seq["mask"], bgr=None)
+    p = config1_params(seed=5)
+    _load(gpu_ctx, kf)
+    gpu_ctx.random_initialization(p)
+    ip, idp, isc = gpu_ctx.get_maps(0)
+    gpu_ctx.red_black_pass(p, 0, 1)
+    gp, gd, gs = gpu_ctx.get_maps(0)
+    ok = or_py.Keyframe.from_synth(kf)
+    op, od, os_ = (np.array(a, copy=True) for a in (ip, idp, isc))
+    r0, r1 = 2000, 2012
+    or_py.propagate_rows(ok, p, op, od, os_, 0, 0, r0 - 3, r1 + 3)
+    or_py.propagate_rows(ok, p, op, od, os_, 0, 1, r0, r1)
+    band = slice(r0, r1)
+    for name, g, o in (("plane", gp[band], op[band]), ("depth", gd[band], od[band]), ("score", gs[band], os_[band])):
+        eq = _report(f"5760x2880 rows {r0}-{r1} {name}", g, o) | (np.isnan(g) & np.isnan(o))
+        assert eq.all(), f"{name}: {np.count_nonzero(~eq)} of {eq.size} differ"
+    assert np.count_nonzero(od[band] != idp[band]) > 0.3 * od[band].size
+
+
 def test_densify_c0_final(gpu_ctx, kf_c0):
     p = config0_params()
     _load(gpu_ctx, kf_c0)
