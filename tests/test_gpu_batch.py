@@ -115,6 +115,24 @@ def test_batch_autotune_keeps_results(gpu_ctx):
     gpu_ctx.batch_set_config()
 
 
+def test_batch_largest_size_with_small(gpu_ctx):
+    """A 5760x2880 keyframe (the largest served size, 4 neighbours) batched
+    with a 960x480 one equals each alone, bit for bit."""
+    from patchmatch_amd import synth
+    kfs = []
+    for seed, (W, H) in ((23, (5760, 2880)), (24, (960, 480))):
+        seq = synth.sequence_gpu(seed, 5, W, H, with_gt=False)
+        ims, poses = seq["images"], seq["poses"]
+        nb = (0, 1, 3, 4) if W == 5760 else (1, 3)
+        kfs.append(synth.Keyframe(W=W, H=H, ref=ims[2], views=[ims[k] for k in nb], ref_pose=poses[2],
+                                  view_poses=np.stack([poses[k] for k in nb]), mask=seq["mask"], gt_depth=None,
+                                  bgr=None))
+    p = config1_params(iterations=1, seed=9)
+    descs = _upload(gpu_ctx, kfs)
+    ref = _plain(gpu_ctx, kfs, p)
+    _assert_same(_batch(gpu_ctx, kfs, descs, p), ref, "5760x2880 + 960x480")
+
+
 def test_batch_rejects_bad_input(gpu_ctx):
     from patchmatch_amd._lib import PmpError
     kfs = _keyframes()
