@@ -6,6 +6,10 @@ hardware byte counters: a copy of B bytes moves B bytes in and B bytes out).
 
     python -m patchmatch_amd.profile_run --steps 2 --copy-mib 256
     python -m patchmatch_amd.profile_run --steps 1 --pipeline 10 --copy-mib 256
+    python -m patchmatch_amd.profile_run --steps 0 --batched 64 --copy-mib 0
+
+--batched n runs the batched workload of the benchmark (n keyframes of mixed
+resolution and view count, plain then batched, workloads.run_batched).
 """
 from __future__ import annotations
 
@@ -27,6 +31,7 @@ def main(argv=None):
     ap.add_argument("--height", type=int, default=960)
     ap.add_argument("--views", type=int, default=6)
     ap.add_argument("--pipeline", type=int, default=0, help="keyframes of the densify -> clean -> fuse trajectory")
+    ap.add_argument("--batched", type=int, default=0, help="keyframes of the batched workload")
     args = ap.parse_args(argv)
 
     from patchmatch_amd import synth
@@ -47,6 +52,12 @@ def main(argv=None):
             ctx.synchronize()
             e, c = ctx.eval_counts()
             print(f"[profile_run] step {s}: {1e3 * (time.perf_counter() - t):.1f} ms, {e} evaluations", flush=True)
+        if args.batched > 0:
+            from patchmatch_amd import workloads
+            st = workloads.Stream(1, workloads.BATCH_CLASSES)
+            r = workloads.run_batched(st, p, args.batched, 2)
+            print(f"[profile_run] batched: {r['plain_depth_maps_per_s']} plain, {r['batched_depth_maps_per_s']} "
+                  f"batched depth maps/s, config {r['autotuned_config']}", flush=True)
         if args.pipeline > 0:
             from patchmatch_amd import workloads
             r = workloads.run_pipeline(5, args.pipeline, args.width, args.height, params=p)
