@@ -556,7 +556,10 @@ int pmp_batch_autotune(pmp_ctx* ctx, const pmp_params* params, int32_t n, const 
         QueueRun r;
         if ((rc = queue_run(ctx, *q, c, zero, false, r))) break;
         const int occ = r.blocks / (q->cus > 0 ? q->cus : 1);   // blocks per CU that fit (rounding aside)
-        for (int bpc = 1; bpc <= occ && rc == 0; ++bpc) {
+        // at least 8 waves per CU (rows waves per block): fewer leave SIMDs
+        // idle and measured 1.3-4x slower, and their trials are the longest
+        const int bpc_min = std::min(occ, (8 + rows_set[ri] - 1) / rows_set[ri]);
+        for (int bpc = bpc_min; bpc <= occ && rc == 0; ++bpc) {
             for (int bands = 0; bands <= 1 && rc == 0; ++bands) {
                 c.blocks_per_cu = bpc;
                 c.xcd_bands = bands;
