@@ -234,6 +234,31 @@ def test_propagate_largest_served_size_row_band(gpu_ctx):
     assert np.count_nonzero(od[band] != idp[band]) > 0.3 * od[band].size
 
 
+def test_plain_rejects_bad_params_and_runs_smallest(gpu_ctx):
+    """Argument errors of the one-keyframe path (patch half, view count,
+    iterations, image below the patch) come back as EINVAL, and the smallest
+    image the 7x7 patch allows (14x14) densifies bit for bit as the oracle."""
+    from patchmatch_amd import synth
+    from patchmatch_amd._lib import PmpError
+    kf = synth.make_keyframe(31, 32, 16, 2)
+    _load(gpu_ctx, kf)
+    for bad in (config0_params(patch_half=11), config0_params(iterations=65), config0_params(patch_half=2)):
+        with pytest.raises(PmpError, match="EINVAL"):
+            gpu_ctx.densify(bad)
+    with pytest.raises(PmpError, match="EINVAL"):
+        gpu_ctx.set_views(0, kf.ref_pose, [1] * 11, [kf.view_poses[0]] * 11)
+    with pytest.raises(PmpError, match="EINVAL"):   # 16 rows < 2 * 5 + 8
+        gpu_ctx.densify(config0_params(patch_half=5))
+    small = synth.make_keyframe(32, 14, 14, 2)
+    p = config0_params(iterations=2, coverage_mode=COVER_FULL)
+    _load(gpu_ctx, small)
+    gpu_ctx.densify(p, small.mask)
+    _, gd, gs, gn = gpu_ctx.get_maps(PMP_MAPS_FINAL)
+    ref = or_py.densify(or_py.Keyframe.from_synth(small), p, small.mask)
+    assert _same_bits(gd, ref["depth"]).all() and _same_bits(gs, ref["score"]).all()
+    assert np.array_equal(gn, ref["nghbr"])
+
+
 def test_densify_c0_final(gpu_ctx, kf_c0):
     p = config0_params()
     _load(gpu_ctx, kf_c0)
