@@ -247,10 +247,24 @@ def _arrivals(rng, rate, n):
     return np.cumsum(rng.exponential(1.0 / rate, n))
 
 
-def _summary(lat, t_first, t_last, n):
+def _summary(lat, t_first, t_last, n, labels=None):
+    """Throughput and latency percentiles; with labels (one per request, e.g.
+    its resolution), the percentiles per label too."""
     lat = np.asarray(lat) * 1e3
-    return {"throughput_per_s": round(n / (t_last - t_first), 3), "p50_ms": round(float(np.percentile(lat, 50)), 1),
-            "p99_ms": round(float(np.percentile(lat, 99)), 1), "max_ms": round(float(lat.max()), 1)}
+    out = {"throughput_per_s": round(n / (t_last - t_first), 3), "p50_ms": round(float(np.percentile(lat, 50)), 1),
+           "p99_ms": round(float(np.percentile(lat, 99)), 1), "max_ms": round(float(lat.max()), 1)}
+    if labels is not None:
+        by = {}
+        for lab in sorted(set(labels)):
+            v = lat[np.asarray(labels) == lab]
+            by[lab] = {"n": int(v.size), "p50_ms": round(float(np.percentile(v, 50)), 1),
+                       "p99_ms": round(float(np.percentile(v, 99)), 1)}
+        out["by_resolution"] = by
+    return out
+
+
+def _labels(stream, specs):
+    return [f"{stream.classes[c][0]}x{stream.classes[c][1]}" for c, _, _ in specs]
 
 
 def run_plain_stream(stream: Stream, params, specs, arrivals):
@@ -276,7 +290,7 @@ def run_plain_stream(stream: Stream, params, specs, arrivals):
         lat.append(t - ta)
         done.append(t)
     ctx.close()
-    return _summary(lat, arrivals[0], max(done), len(specs))
+    return _summary(lat, arrivals[0], max(done), len(specs), _labels(stream, specs))
 
 
 def run_served_stream(stream: Stream, params, specs, arrivals, max_slots=16, hbm_budget=64 << 30, inflight_kpx=0):
@@ -325,7 +339,7 @@ def run_served_stream(stream: Stream, params, specs, arrivals, max_slots=16, hbm
         done.append(d)
     st = srv.stats()
     srv.close()
-    out = _summary(lat, arrivals[0], max(done), len(specs))
+    out = _summary(lat, arrivals[0], max(done), len(specs), _labels(stream, specs))
     out.update({"max_in_flight": st["max_in_flight"], "uploads": st["uploads"], "evictions": st["evictions"],
                 "steps": st["steps"]})
     return out
