@@ -187,15 +187,14 @@ static inline float or_atan2(float y, float x) {
 /* longitude atan2(x, z) and latitude atan2(y, h), h = sqrt(x^2 + z^2), of a
  * direction with finite components (|x|, |y|, |z| < 1e7): rs = rsqrt(h^2 y^2)
  * gives |y|/h = y^2 rs and h/|y| = h^2 rs; the latitude is atan of the
- * smaller (pi/2 minus it above 45 degrees).  h^2 clamped to >= 1e-24,
- * h^2 y^2 to [1e-30, 1e30]. */
+ * smaller (pi/2 minus it above 45 degrees, i.e. when y^2 > h^2).  h^2
+ * clamped to >= 1e-24, h^2 y^2 to [1e-30, 1e30]. */
 static inline void or_lonlat(float x, float y, float z, float* lon, float* lat) {
     float h2 = or_max(fmaf(x, x, z * z), 1e-24f);
     float y2 = y * y;
     float rs = or_rsqrt_pos(or_med3(h2 * y2, 1e-30f, 1e30f));
-    float t = y2 * rs, tp = h2 * rs;
-    int big = t > tp;
-    float a = or_atan_core(or_med3(big ? tp : t, 0.0f, 1.0f));
+    int big = y2 > h2;
+    float a = or_atan_core(or_med3((big ? h2 : y2) * rs, 0.0f, 1.0f));
     if (big) a = OR_PIO2_F - a;
     *lat = copysignf(a, y);
     *lon = or_atan2(x, z);

@@ -155,15 +155,15 @@ PMP_DF float atan2_(float y, float x) {
 // Longitude atan2(x, z) and latitude atan2(y, h), h = sqrt(x^2 + z^2), of a
 // direction (x, y, z) with finite components (|x|, |y|, |z| < 1e7).  One
 // rsqrt of h^2 y^2 gives both ratios, |y| / h = y^2 rs and h / |y| = h^2 rs;
-// the latitude is atan of the smaller one (pi/2 minus it above 45 degrees).
+// the latitude is atan of the smaller one (pi/2 minus it above 45 degrees,
+// i.e. when y^2 > h^2, which picks the ratio before it is formed).
 // h^2 is clamped to >= 1e-24, h^2 y^2 to [1e-30, 1e30].
 PMP_DF void lonlat(float x, float y, float z, float& lon, float& lat) {
     const float h2 = max_(fma_(x, x, z * z), 1e-24f);
     const float y2 = y * y;
     const float rs = rsqrt_pos(med3_(h2 * y2, 1e-30f, 1e30f));
-    const float t = y2 * rs, tp = h2 * rs;
-    const bool big = t > tp;
-    float a = atan_core(med3_(big ? tp : t, 0.0f, 1.0f));
+    const bool big = y2 > h2;
+    float a = atan_core(med3_((big ? h2 : y2) * rs, 0.0f, 1.0f));
     if (big) a = kPiO2 - a;
     lat = __builtin_copysignf(a, y);
     lon = atan2_(x, z);
