@@ -171,13 +171,15 @@ static inline float or_med3(float a, float b, float c) {
     return (lo < m) ? m : lo;
 }
 
-/* atan2(y, x) for finite y, x; result in [-pi, pi] with the sign of y
- * (atan2(+-0, x >= 0) = +-0, atan2(+-0, x < 0) = +-pi) */
+/* atan2(y, x) for |y|, |x| <= 1e30; result in [-pi, pi] with the sign of y
+ * (atan2(+-0, x >= 0) = +-0, atan2(+-0, x < 0) = +-pi).  mn / mx is not
+ * clamped: x or_rcp_pos(x) <= 1 for every x in [1, 2), the seed and steps scale
+ * exactly with the binade, and rounding is monotone (pinned by the tests). */
 static inline float or_atan2(float y, float x) {
     float ay = fabsf(y), ax = fabsf(x);
     float mn = or_med3(ay, ax, 0.0f);                              /* min of two non-negatives */
     float mx = or_med3(or_med3(ay, ax, 1e30f), 1e-30f, 1e30f);     /* max, clamped */
-    float q = or_med3(mn * or_rcp_pos(mx), 0.0f, 1.0f);
+    float q = mn * or_rcp_pos(mx);                                 /* in [0, 1] */
     float a = or_atan_core(q);
     if (ay > ax) a = OR_PIO2_F - a;
     if (x < 0.0f) a = OR_PI_F - a;
@@ -187,14 +189,15 @@ static inline float or_atan2(float y, float x) {
 /* longitude atan2(x, z) and latitude atan2(y, h), h = sqrt(x^2 + z^2), of a
  * direction with finite components (|x|, |y|, |z| < 1e7): rs = rsqrt(h^2 y^2)
  * gives |y|/h = y^2 rs and h/|y| = h^2 rs; the latitude is atan of the
- * smaller (pi/2 minus it above 45 degrees, i.e. when y^2 > h^2).  h^2
- * clamped to >= 1e-24, h^2 y^2 to [1e-30, 1e30]. */
+ * smaller (pi/2 minus it above 45 degrees, i.e. when y^2 > h^2).  h^2 y^2
+ * is clamped to [1e-30, 1e30]; the ratio is not clamped (it exceeds 1 only
+ * when y^2 = h^2 exactly, by 2^-23 at most). */
 static inline void or_lonlat(float x, float y, float z, float* lon, float* lat) {
-    float h2 = or_max(fmaf(x, x, z * z), 1e-24f);
+    float h2 = fmaf(x, x, z * z);
     float y2 = y * y;
     float rs = or_rsqrt_pos(or_med3(h2 * y2, 1e-30f, 1e30f));
     int big = y2 > h2;
-    float a = or_atan_core(or_med3((big ? h2 : y2) * rs, 0.0f, 1.0f));
+    float a = or_atan_core((big ? h2 : y2) * rs);
     if (big) a = OR_PIO2_F - a;
     *lat = copysignf(a, y);
     *lon = or_atan2(x, z);

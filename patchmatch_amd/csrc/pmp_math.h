@@ -139,13 +139,16 @@ PMP_DF float max_(float a, float b) { return __builtin_fmaxf(a, b); }
 // [0, K]; unlike fminf/fmaxf it needs no quieting of its operands.
 PMP_DF float med3_(float a, float b, float c) { return __builtin_amdgcn_fmed3f(a, b, c); }
 
-// atan2(y, x) for finite arguments, in [-pi, pi], with the sign of y
-// (atan2(+-0, x >= 0) = +-0, atan2(+-0, x < 0) = +-pi)
+// atan2(y, x) for |y|, |x| <= 1e30, in [-pi, pi], with the sign of y
+// (atan2(+-0, x >= 0) = +-0, atan2(+-0, x < 0) = +-pi).  The ratio min/max
+// needs no clamp to [0, 1]: rcp_pos's seed and steps scale exactly with the
+// binade, m rcp_pos(m) <= 1 for every float m in [1, 2) (checked exhaustively),
+// and rounding is monotone, so mn rcp_pos(mx) <= mx rcp_pos(mx) <= 1.
 PMP_DF float atan2_(float y, float x) {
     const float ay = abs_(y), ax = abs_(x);
     const float mn = med3_(ay, ax, 0.0f);                           // min of two non-negatives
     const float mx = med3_(med3_(ay, ax, 1e30f), 1e-30f, 1e30f);    // max, clamped
-    const float q = med3_(mn * rcp_pos(mx), 0.0f, 1.0f);
+    const float q = mn * rcp_pos(mx);
     float a = atan_core(q);
     if (ay > ax) a = kPiO2 - a;
     if (x < 0.0f) a = kPi - a;
@@ -157,13 +160,17 @@ PMP_DF float atan2_(float y, float x) {
 // rsqrt of h^2 y^2 gives both ratios, |y| / h = y^2 rs and h / |y| = h^2 rs;
 // the latitude is atan of the smaller one (pi/2 minus it above 45 degrees,
 // i.e. when y^2 > h^2, which picks the ratio before it is formed).
-// h^2 is clamped to >= 1e-24, h^2 y^2 to [1e-30, 1e30].
+// h^2 y^2 is clamped to [1e-30, 1e30].  The ratio is not clamped: for
+// components below 1e7 the product stays below 1e30, a clamp raising a tiny
+// product only lowers the ratio, and the ratio exceeds 1 only when y^2 = h^2
+// exactly, by 2^-23 at most (atan_core is then pi/4 within an ulp).  h = 0
+// gives the ratio 0 and the latitude +-pi/2; y = 0 the latitude +-0.
 PMP_DF void lonlat(float x, float y, float z, float& lon, float& lat) {
-    const float h2 = max_(fma_(x, x, z * z), 1e-24f);
+    const float h2 = fma_(x, x, z * z);
     const float y2 = y * y;
     const float rs = rsqrt_pos(med3_(h2 * y2, 1e-30f, 1e30f));
     const bool big = y2 > h2;
-    float a = atan_core(med3_((big ? h2 : y2) * rs, 0.0f, 1.0f));
+    float a = atan_core((big ? h2 : y2) * rs);
     if (big) a = kPiO2 - a;
     lat = __builtin_copysignf(a, y);
     lon = atan2_(x, z);

@@ -81,6 +81,24 @@ def test_atan2_vs_libm():
     assert z[0] == np.float32(-np.pi) and z[1] == 0.0 and np.signbit(z[1]) and z[2] == np.float32(np.pi)
 
 
+def test_atan2_ratio_needs_no_clamp():
+    """atan2 and the latitude form their ratio in [0, 1] without a clamp:
+    m rcp(m) <= 1 for every float m in [1, 2) (the reciprocal's seed and steps
+    scale exactly with the binade; monotone rounding covers mn < mx), and
+    m rsqrt(m M) exceeds 1 only for M = m, by one ulp."""
+    m = (np.arange(1 << 23, dtype=np.uint32) | np.uint32(0x3F800000)).view(np.float32)
+    assert (m * or_py.eval_math("rcp", m)).astype(np.float32).max() <= 1.0
+    for lo in (0x3F800000, 0x40000000):     # rsqrt repeats every two binades
+        m = (np.arange(1 << 23, dtype=np.uint32) | np.uint32(lo)).view(np.float32)
+        nxt = np.nextafter(m, np.float32(np.inf))
+        assert (m * or_py.eval_math("rsqrt", (m * nxt).astype(np.float32))).astype(np.float32).max() <= 1.0
+        assert (m * or_py.eval_math("rsqrt", (m * m).astype(np.float32))).astype(np.float32).max() \
+            <= np.float32(1.0) + np.float32(2.0 ** -23)
+    # both poles and the horizon without a clamp on h^2: (b, a, b) with b = 0
+    lat = or_py.eval_math("lat", np.float32([2.0, -3.0, 0.0]), np.float32([0.0, 0.0, 0.0]))
+    assert lat[0] == np.float32(np.pi / 2) and lat[1] == -np.float32(np.pi / 2) and lat[2] == 0.0
+
+
 def test_latitude_vs_libm():
     """Latitude of the direction (x, y, z) = (b, a, b): atan2(a, sqrt(2) |b|),
     over both branches (|y| below and above the horizontal norm) and the poles."""
