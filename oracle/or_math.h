@@ -178,7 +178,7 @@ static inline float or_med3(float a, float b, float c) {
 static inline float or_atan2(float y, float x) {
     float ay = fabsf(y), ax = fabsf(x);
     float mn = or_med3(ay, ax, 0.0f);                              /* min of two non-negatives */
-    float mx = or_med3(or_med3(ay, ax, 1e30f), 1e-30f, 1e30f);     /* max, clamped */
+    float mx = or_max(or_max(ay, ax), 1e-30f);                     /* max, floored */
     float q = mn * or_rcp_pos(mx);                                 /* in [0, 1] */
     float a = or_atan_core(q);
     if (ay > ax) a = OR_PIO2_F - a;

@@ -140,14 +140,16 @@ PMP_DF float max_(float a, float b) { return __builtin_fmaxf(a, b); }
 PMP_DF float med3_(float a, float b, float c) { return __builtin_amdgcn_fmed3f(a, b, c); }
 
 // atan2(y, x) for |y|, |x| <= 1e30, in [-pi, pi], with the sign of y
-// (atan2(+-0, x >= 0) = +-0, atan2(+-0, x < 0) = +-pi).  The ratio min/max
+// (atan2(+-0, x >= 0) = +-0, atan2(+-0, x < 0) = +-pi).  The divisor max(|y|,
+// |x|) is floored at 1e-30 and, within that range, needs no upper clamp for
+// rcp_pos (valid up to 2^125).  The ratio min/max
 // needs no clamp to [0, 1]: rcp_pos's seed and steps scale exactly with the
 // binade, m rcp_pos(m) <= 1 for every float m in [1, 2) (checked exhaustively),
 // and rounding is monotone, so mn rcp_pos(mx) <= mx rcp_pos(mx) <= 1.
 PMP_DF float atan2_(float y, float x) {
     const float ay = abs_(y), ax = abs_(x);
     const float mn = med3_(ay, ax, 0.0f);                           // min of two non-negatives
-    const float mx = med3_(med3_(ay, ax, 1e30f), 1e-30f, 1e30f);    // max, clamped
+    const float mx = max_(max_(ay, ax), 1e-30f);                    // max, floored (one v_max3_f32)
     const float q = mn * rcp_pos(mx);
     float a = atan_core(q);
     if (ay > ax) a = kPiO2 - a;

@@ -49,8 +49,10 @@ MATH_CASES = {
 def test_math_bitwise(gpu_ctx, name):
     rng = np.random.default_rng(11)
     if name == "atan2":
-        a = np.concatenate([rng.normal(0, 10, 300000), [0.0, 0.0, -1.0, 1e-35, 5.0, -0.0, -0.0]]).astype(np.float32)
-        b = np.concatenate([rng.normal(0, 10, 300000), [0.0, -1.0, 0.0, 1e-35, -5.0, -1.0, 1.0]]).astype(np.float32)
+        a = np.concatenate([rng.normal(0, 10, 300000), [0.0, 0.0, -1.0, 1e-35, 5.0, -0.0, -0.0],
+                            [1e30, -1e-40, 3e29, -1e30]]).astype(np.float32)
+        b = np.concatenate([rng.normal(0, 10, 300000), [0.0, -1.0, 0.0, 1e-35, -5.0, -1.0, 1.0],
+                            [1e-40, 1e30, -3e29, -1e30]]).astype(np.float32)
     elif name == "lat":    # latitude of the direction (b, a, b)
         a = np.concatenate([rng.normal(0, 10, 300000), rng.normal(0, 1e-3, 1000),
                             [0.0, -0.0, 1.0, -1.0, 1e-20, 7.0]]).astype(np.float32)
