@@ -44,7 +44,7 @@ HBM_PEAK_GBS = 8000.0
 # L2-to-memory bytes per k_propagate_q launch on the 1920x960 + 6-view
 # workload, from rocprofv3 FETCH_SIZE and WRITE_SIZE scaled by a copy of known
 # size (profiles/k_propagate_q_c1_1920x960_6views.md); None for other workloads.
-TRAFFIC_BYTES_PER_LAUNCH_C1 = int((366_750 * 2.00 + 42_982 * 1.00) * 1024)
+TRAFFIC_BYTES_PER_LAUNCH_C1 = int((370_600 * 2.00 + 42_982 * 1.00) * 1024)
 
 
 def ops_per_eval(T: int, N: int) -> int:
