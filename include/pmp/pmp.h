@@ -44,7 +44,14 @@ enum {
     PMP_EBUSY = -6       /* not finished within the time limit */
 };
 
-enum { PMP_AGG_REF_FOLD = 0, PMP_AGG_MAX = 1, PMP_AGG_TOPK_MEAN = 2 };
+/* PMP_AGG_REF_FOLD: the fold of PatchMatchCuda.cu:161-171 with the moment sums
+ * pooled across views, as there (zeroed once at :57-62, outside the view loop
+ * :101), so view v's NCC is taken over views 0..v together;
+ * PMP_AGG_MAX: maximum of per-view NCCs (ComputePlaneScore, depthmap.cc:1104-1127);
+ * PMP_AGG_TOPK_MEAN: mean of the topk largest per-view NCCs;
+ * PMP_AGG_FOLD_INDEP: the fold over independent per-view NCCs (no counterpart
+ * in the reference). */
+enum { PMP_AGG_REF_FOLD = 0, PMP_AGG_MAX = 1, PMP_AGG_TOPK_MEAN = 2, PMP_AGG_FOLD_INDEP = 3 };
 enum { PMP_EDGE_WRAP = 0, PMP_EDGE_CLAMP = 1, PMP_EDGE_ZERO = 2 };
 enum { PMP_COVER_REF_GRID = 0, PMP_COVER_FULL = 1 };
 enum { PMP_INIT_REF_UNIFORM = 0, PMP_INIT_INVDEPTH_HEMI = 1 };
@@ -60,7 +67,7 @@ typedef struct pmp_params {
     int32_t patch_stride;      /* 2: PatchMatchCuda.cu:104-106 */
     int32_t iterations;        /* red-black iterations (Dense.Iterations) */
     int32_t n_refine;          /* 6 random refinements per pass (:363-395), <= 8 */
-    int32_t agg_mode;          /* PMP_AGG_*: view aggregation (:170-171 fold) */
+    int32_t agg_mode;          /* PMP_AGG_*: view aggregation (:161-171 pooled fold) */
     int32_t topk;              /* k of PMP_AGG_TOPK_MEAN */
     int32_t edge_mode;         /* PMP_EDGE_*: neighbour sampling outside the image */
     int32_t coverage_mode;     /* PMP_COVER_*: reference launch grids or every interior pixel */
@@ -81,7 +88,8 @@ typedef struct pmp_params {
     float refine_depth_decay;  /* 0.3 */
     float refine_normal_decay; /* 0.8 */
     float min_patch_sd;        /* 0: low-texture gate (depthmap.cc:739) */
-    float median_rel_thr;      /* 0.05: PostProcess median test */
+    double median_rel_thr;     /* 0.05: PostProcess median test, a double as the literal
+                                  of depthmap.cc:1190 (the float ratio is compared with it) */
 } pmp_params;
 
 typedef struct pmp_pose {

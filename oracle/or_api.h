@@ -18,7 +18,10 @@
 #define OR_MAX_TAPS 121
 #define OR_MAX_REFINE 8
 
-enum { OR_AGG_REF_FOLD = 0, OR_AGG_MAX = 1, OR_AGG_TOPK_MEAN = 2 };
+/* REF_FOLD: the device fold of PatchMatchCuda.cu:161-171 over moments pooled
+ * across views (:57-62 zero them once, outside the view loop :101);
+ * FOLD_INDEP: the same fold over independent per-view NCCs */
+enum { OR_AGG_REF_FOLD = 0, OR_AGG_MAX = 1, OR_AGG_TOPK_MEAN = 2, OR_AGG_FOLD_INDEP = 3 };
 enum { OR_EDGE_WRAP = 0, OR_EDGE_CLAMP = 1, OR_EDGE_ZERO = 2 };
 enum { OR_COVER_REF_GRID = 0, OR_COVER_FULL = 1 };
 enum { OR_INIT_REF_UNIFORM = 0, OR_INIT_INVDEPTH_HEMI = 1 };
@@ -33,7 +36,8 @@ typedef struct {
     float min_score, sigma_color, sigma_space, var_floor, fold_agree;
     float depth_penalty, plane_penalty;
     float refine_depth_range, refine_normal_range, refine_depth_decay, refine_normal_decay;
-    float min_patch_sd, median_rel_thr;
+    float min_patch_sd;
+    double median_rel_thr;   /* compared in double, as the literal 0.05 of depthmap.cc:1190 */
 } or_params;
 
 typedef struct {

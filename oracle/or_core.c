@@ -29,6 +29,11 @@
  *     centred on their weighted mean, neighbour samples shifted by the
  *     view's first-tap sample;
  *   - explicit neighbour bilinear with longitude wrap / latitude clamp;
+ *   - the reference fold (REF_FOLD) over moments pooled across views, as
+ *     the device accumulates them (:57-62 zeroed once, outside the view loop
+ *     :101): view v's NCC is that of views 0..v together, formed from the
+ *     per-view shifted moments by running sums of the covariances and
+ *     variances plus Welford's update of the spread of the view means;
  *   - NCC clamped to [-1, 1]; the fold's agreement test written without the
  *     division (|s - s_v| < agree |s_v|, identical at s_v = 0);
  *   - counter-based Philox4x32-10 random streams instead of clock-seeded
@@ -64,6 +69,7 @@ typedef struct {
     float ku, hu, kv, hv;
     float dr[OR_MAX_REFINE], nr[OR_MAX_REFINE];
     float inv_k, inv_T;
+    float rn[OR_MAX_VIEWS];   /* 1/(v+1), rounded once */
     int topk;
     uint32_t key[2];
 } derived_t;
@@ -152,6 +158,7 @@ static int derive(const or_keyframe* kf, const or_params* p, derived_t* d) {
     d->topk = k;
     d->inv_k = (float)(1.0 / (double)k);
     d->inv_T = (float)(1.0 / (double)T);
+    for (int v = 0; v < OR_MAX_VIEWS; ++v) d->rn[v] = (float)(1.0 / (double)(v + 1));
     d->key[0] = (uint32_t)(p->seed & 0xFFFFFFFFu);
     d->key[1] = (uint32_t)(p->seed >> 32);
     return 0;
@@ -293,7 +300,47 @@ static inline float ncc_f(const refstat_t* rs, float floor_, float Swy, float Sw
     return or_clamp(s, -1.0f, 1.0f);
 }
 
-/* view aggregation: fold PatchMatchCuda.cu:170-171 / max depthmap.cc:1110-1117 / top-k mean */
+/* the fold of PatchMatchCuda.cu:171: s <- (|(s - s2)/s2| < agree) ? (s + s2)/2 : max(s, s2) */
+static inline float fold_f(float s, float s2, float agree) {
+    if (fabsf(s - s2) < agree * fabsf(s2)) return 0.5f * (s + s2);
+    return (s2 > s) ? s2 : s;
+}
+
+/* Moments pooled across views (REF_FOLD).  PatchMatchCuda.cu:57-62 zero the
+ * six sums once and the view loop (:101-157) keeps adding to them, so the
+ * NCC of view v at :161-169 is that of the taps of views 0..v together: the
+ * reference side repeats, so its mean and variance stay those of one view;
+ * the pooled covariance is the mean of the per-view covariances; the pooled
+ * variance of y is the mean of the per-view variances plus the spread of the
+ * per-view means.  C and V sum the covariances and variances, mu and q are
+ * Welford's running mean and sum of squared deviations of the view means. */
+typedef struct { float C, V, mu, q, s; } pool_t;
+
+static inline void pool_init(pool_t* st) { st->C = 0.0f; st->V = 0.0f; st->mu = 0.0f; st->q = 0.0f; st->s = -1.0f; }
+
+/* adds view v (shifted moments acc0..2 about its shift), returns the pooled NCC
+ * of views 0..v and folds it into st->s */
+static inline float pool_view(pool_t* st, const refstat_t* rs, float floor_, float agree, float rn, float acc0,
+                              float acc1, float acc2, float shift) {
+    float my = acc0 * rs->inv;
+    float vy = fmaf(-my, my, acc1 * rs->inv);
+    float cov = fmaf(-rs->mxc, my, acc2 * rs->inv);
+    float m = shift + my;
+    st->C = st->C + cov;
+    st->V = st->V + vy;
+    float dm = m - st->mu;
+    st->mu = fmaf(dm, rn, st->mu);
+    st->q = fmaf(dm, m - st->mu, st->q);
+    float cp = st->C * rn;
+    float vp = (st->V + st->q) * rn;
+    float s2 = -1.0f;
+    if (!(rs->vx < floor_ || vp < floor_)) s2 = or_clamp(cp * or_rsqrt_pos(rs->vx * vp), -1.0f, 1.0f);
+    st->s = fold_f(st->s, s2, agree);
+    return s2;
+}
+
+/* view aggregation of independent per-view scores: fold PatchMatchCuda.cu:170-171
+ * (REF_FOLD given the pooled scores, FOLD_INDEP) / max depthmap.cc:1110-1117 / top-k mean */
 static inline float aggregate_f(const derived_t* d, const or_params* p, const float* sv) {
     int N = d->N;
     if (p->agg_mode == OR_AGG_MAX) {
@@ -315,12 +362,7 @@ static inline float aggregate_f(const derived_t* d, const or_params* p, const fl
         return sum * d->inv_k;
     }
     float s = -1.0f;
-    float agree = p->fold_agree;
-    for (int v = 0; v < N; ++v) {
-        float s2 = sv[v];
-        if (fabsf(s - s2) < agree * fabsf(s2)) s = 0.5f * (s + s2);
-        else s = (s2 > s) ? s2 : s;
-    }
+    for (int v = 0; v < N; ++v) s = fold_f(s, sv[v], p->fold_agree);
     return s;
 }
 
@@ -355,6 +397,14 @@ static float score_plane_f(const derived_t* d, const or_params* p, const or_keyf
         }
     }
     float sv[OR_MAX_VIEWS];
+    if (p->agg_mode == OR_AGG_REF_FOLD) {
+        pool_t st;
+        pool_init(&st);
+        for (int v = 0; v < N; ++v)
+            sv[v] = pool_view(&st, rs, p->var_floor, p->fold_agree, d->rn[v], acc0[v], acc1[v], acc2[v], shift[v]);
+        if (view_scores) for (int v = 0; v < N; ++v) view_scores[v] = sv[v];
+        return st.s;
+    }
     for (int v = 0; v < N; ++v) sv[v] = ncc_f(rs, p->var_floor, acc0[v], acc1[v], acc2[v]);
     if (view_scores) for (int v = 0; v < N; ++v) view_scores[v] = sv[v];
     return aggregate_f(d, p, sv);
@@ -459,7 +509,8 @@ EXPORT int or_init(const or_keyframe* kf, const or_params* p, float* plane4, flo
 }
 
 /* score of a fixed plane field at every interior pixel (ScorePlaneDepth_, :255-270);
- * non-interior pixels get NaN; view_scores (nullable) gets the N per-view NCC */
+ * non-interior pixels get NaN; view_scores (nullable) gets the N per-view NCC
+ * (REF_FOLD: view v's NCC over the moments pooled over views 0..v) */
 EXPORT int or_score_field(const or_keyframe* kf, const or_params* p, const float* plane4, float* score,
                           float* view_scores) {
     derived_t d;
@@ -670,7 +721,9 @@ EXPORT int or_finalize(const or_keyframe* kf, const or_params* p, const uint8_t*
             for (int j = 0; j < W; ++j) {
                 size_t k = (size_t)i * W + j;
                 float dd = out_depth[k], m = m2[k], sc = out_score[k];
-                if (dd == 0.0f || (double)(fabsf(dd - m) / dd) > (double)p->median_rel_thr ||
+                /* the float quotient against the double threshold (:1190: fabs of a
+                 * float is float there, 0.05 a double literal) */
+                if (dd == 0.0f || (double)(fabsf(dd - m) / dd) > p->median_rel_thr ||
                     dd < p->min_depth || dd > p->max_depth || sc <= p->min_score) {
                     out_depth[k] = 0.0f; out_score[k] = -1.0f;
                 } else {
@@ -974,14 +1027,94 @@ static double sample_d(const uint8_t* img, int W, int H, int edge, double u, dou
     return top + fy * (bot - top);
 }
 
+/* exact (u, v) of tap t of pixel (i, j) through plane p in view v: bearings
+ * from double trig, the two-step rigid transform of PatchMatchCuda.cu:117-130
+ * (|b.p| clamped to >= 1e-6), libm atan2/asin (:132-144) */
+static void warp_tap_d(const or_keyframe* kf, const derived_t* d, int ii, int jj, int v, double px, double py,
+                       double pz, double* uo, double* vo) {
+    int W = d->W, H = d->H;
+    const double* R0 = kf->ref_pose; const double* t0 = kf->ref_pose + 9;
+    const double* Rn = kf->view_poses + 12 * v; const double* tn = Rn + 9;
+    double lon = 2.0 * M_PI * ((double)jj / W - 0.5);
+    double lat = -M_PI * ((double)ii / H - 0.5);
+    double bx = cos(lat) * sin(lon), by = -sin(lat), bz = cos(lat) * cos(lon);
+    double s = px * bx + py * by + pz * bz;
+    double as = fabs(s); if (!(as >= 1e-6)) as = 1e-6;
+    if (s < 0) as = -as;
+    double X[3] = { bx / as - t0[0], by / as - t0[1], bz / as - t0[2] };
+    double Xw[3];
+    for (int r = 0; r < 3; ++r) Xw[r] = R0[r] * X[0] + R0[3 + r] * X[1] + R0[6 + r] * X[2];
+    double Xn[3];
+    for (int r = 0; r < 3; ++r) Xn[r] = Rn[3 * r] * Xw[0] + Rn[3 * r + 1] * Xw[1] + Rn[3 * r + 2] * Xw[2] + tn[r];
+    double rr = sqrt(Xn[0] * Xn[0] + Xn[1] * Xn[1] + Xn[2] * Xn[2]);
+    if (rr < 1e-6) rr = 1e-6;
+    double syv = Xn[1] / rr;
+    if (syv > 1) syv = 1;
+    if (syv < -1) syv = -1;
+    *uo = W * (0.5 + atan2(Xn[0], Xn[2]) / (2.0 * M_PI));
+    *vo = H * (0.5 + asin(syv) / M_PI);
+}
+
+/* FP64 score of plane (px, py, pz) at pixel (i, j): single-exp bilateral
+ * weight (:149), raw one-pass moments (:150-169; for REF_FOLD the sums run on
+ * across views, :57-62, :101), NCC with the variance floor, aggregation.
+ * uv32 (nullable): sample view v's tap t at the given FP32 coordinates
+ * uv32[2 (v T + t)], instead of the exact warp. */
+static double score_pixel_d(const or_keyframe* kf, const or_params* p, const derived_t* d, int i, int j, double px,
+                            double py, double pz, const float* uv32, double* view_scores) {
+    int W = d->W, H = d->H, N = d->N;
+    double cf = 1.0 / (2.0 * (double)p->sigma_color * p->sigma_color);
+    double xf = 1.0 / (2.0 * (double)p->sigma_space * p->sigma_space);
+    double Ic = kf->ref[(size_t)i * W + j];
+    double sv[OR_MAX_VIEWS];
+    const int pooled = p->agg_mode == OR_AGG_REF_FOLD;
+    double sx = 0, sy_ = 0, sxx = 0, syy = 0, sxy = 0, sw = 0;
+    for (int v = 0; v < N; ++v) {
+        if (!pooled) { sx = 0; sy_ = 0; sxx = 0; syy = 0; sxy = 0; sw = 0; }
+        for (int t = 0; t < d->T; ++t) {
+            int ii = i + d->k1[t], jj = j + d->k2[t];
+            double u, vv;
+            if (uv32) { u = uv32[2 * (v * d->T + t)]; vv = uv32[2 * (v * d->T + t) + 1]; }
+            else warp_tap_d(kf, d, ii, jj, v, px, py, pz, &u, &vv);
+            double y = sample_d(kf->views[v], W, H, p->edge_mode, u, vv);
+            double x = kf->ref[(size_t)ii * W + jj];
+            double w = exp(-(x - Ic) * (x - Ic) * cf - (double)(d->k1[t] * d->k1[t] + d->k2[t] * d->k2[t]) * xf);
+            sx += w * x; sy_ += w * y; sxx += w * x * x; syy += w * y * y; sxy += w * x * y; sw += w;
+        }
+        double mx = sx / sw, my = sy_ / sw;
+        double vx = sxx / sw - mx * mx, vy = syy / sw - my * my;
+        double c = (vx < p->var_floor || vy < p->var_floor) ? -1.0 : (sxy / sw - mx * my) / sqrt(vx * vy);
+        if (c > 1) c = 1;
+        if (c < -1) c = -1;
+        sv[v] = c;
+        if (view_scores) view_scores[v] = c;
+    }
+    double s;
+    if (p->agg_mode == OR_AGG_MAX) {
+        s = -1; for (int v = 0; v < N; ++v) if (sv[v] > s) s = sv[v];
+    } else if (p->agg_mode == OR_AGG_TOPK_MEAN) {
+        double tmp[OR_MAX_VIEWS]; for (int v = 0; v < N; ++v) tmp[v] = sv[v];
+        s = 0;
+        for (int a = 0; a < d->topk; ++a) {
+            int best = a; for (int b = a + 1; b < N; ++b) if (tmp[b] > tmp[best]) best = b;
+            double t = tmp[a]; tmp[a] = tmp[best]; tmp[best] = t; s += tmp[a];
+        }
+        s /= d->topk;
+    } else {   /* REF_FOLD (pooled per-view scores), FOLD_INDEP */
+        s = -1;
+        for (int v = 0; v < N; ++v) {
+            if (fabs(s - sv[v]) < p->fold_agree * fabs(sv[v])) s = 0.5 * (s + sv[v]);
+            else if (sv[v] > s) s = sv[v];
+        }
+    }
+    return s;
+}
+
 EXPORT int or_score_field_f64(const or_keyframe* kf, const or_params* p, const float* plane4, double* score,
                               double* view_scores) {
     derived_t d;
     if (derive(kf, p, &d)) return -1;
     int W = d.W, H = d.H, N = d.N;
-    const double* R0 = kf->ref_pose; const double* t0 = kf->ref_pose + 9;
-    double cf = 1.0 / (2.0 * (double)p->sigma_color * p->sigma_color);
-    double xf = 1.0 / (2.0 * (double)p->sigma_space * p->sigma_space);
 #pragma omp parallel for schedule(dynamic, 1)
     for (int i = 0; i < H; ++i)
         for (int j = 0; j < W; ++j) {
@@ -991,67 +1124,67 @@ EXPORT int or_score_field_f64(const or_keyframe* kf, const or_params* p, const f
                 if (view_scores) for (int v = 0; v < N; ++v) view_scores[k * N + v] = NAN;
                 continue;
             }
-            double px = plane4[4 * k], py = plane4[4 * k + 1], pz = plane4[4 * k + 2];
-            double Ic = kf->ref[k];
-            double sv[OR_MAX_VIEWS];
-            for (int v = 0; v < N; ++v) {
-                const double* Rn = kf->view_poses + 12 * v; const double* tn = Rn + 9;
-                double sx = 0, sy_ = 0, sxx = 0, syy = 0, sxy = 0, sw = 0;
-                for (int t = 0; t < d.T; ++t) {
-                    int ii = i + d.k1[t], jj = j + d.k2[t];
-                    double lon = 2.0 * M_PI * ((double)jj / W - 0.5);
-                    double lat = -M_PI * ((double)ii / H - 0.5);
-                    double bx = cos(lat) * sin(lon), by = -sin(lat), bz = cos(lat) * cos(lon);
-                    double s = px * bx + py * by + pz * bz;
-                    double as = fabs(s); if (!(as >= 1e-6)) as = 1e-6;
-                    if (s < 0) as = -as;
-                    double X[3] = { bx / as - t0[0], by / as - t0[1], bz / as - t0[2] };
-                    double Xw[3];
-                    for (int r = 0; r < 3; ++r) Xw[r] = R0[r] * X[0] + R0[3 + r] * X[1] + R0[6 + r] * X[2];
-                    double Xn[3];
-                    for (int r = 0; r < 3; ++r) Xn[r] = Rn[3 * r] * Xw[0] + Rn[3 * r + 1] * Xw[1] + Rn[3 * r + 2] * Xw[2] + tn[r];
-                    double rr = sqrt(Xn[0] * Xn[0] + Xn[1] * Xn[1] + Xn[2] * Xn[2]);
-                    if (rr < 1e-6) rr = 1e-6;
-                    double syv = Xn[1] / rr;
-                    if (syv > 1) syv = 1;
-                    if (syv < -1) syv = -1;
-                    double u = W * (0.5 + atan2(Xn[0], Xn[2]) / (2.0 * M_PI));
-                    double vv = H * (0.5 + asin(syv) / M_PI);
-                    double y = sample_d(kf->views[v], W, H, p->edge_mode, u, vv);
-                    double x = kf->ref[(size_t)ii * W + jj];
-                    double w = exp(-(x - Ic) * (x - Ic) * cf - (double)(d.k1[t] * d.k1[t] + d.k2[t] * d.k2[t]) * xf);
-                    sx += w * x; sy_ += w * y; sxx += w * x * x; syy += w * y * y; sxy += w * x * y; sw += w;
-                }
-                double mx = sx / sw, my = sy_ / sw;
-                double vx = sxx / sw - mx * mx, vy = syy / sw - my * my;
-                double c = (vx < p->var_floor || vy < p->var_floor) ? -1.0 : (sxy / sw - mx * my) / sqrt(vx * vy);
-                if (c > 1) c = 1;
-                if (c < -1) c = -1;
-                sv[v] = c;
-                if (view_scores) view_scores[k * N + v] = c;
-            }
-            double s;
-            if (p->agg_mode == OR_AGG_MAX) {
-                s = -1; for (int v = 0; v < N; ++v) if (sv[v] > s) s = sv[v];
-            } else if (p->agg_mode == OR_AGG_TOPK_MEAN) {
-                double tmp[OR_MAX_VIEWS]; for (int v = 0; v < N; ++v) tmp[v] = sv[v];
-                s = 0;
-                for (int a = 0; a < d.topk; ++a) {
-                    int best = a; for (int b = a + 1; b < N; ++b) if (tmp[b] > tmp[best]) best = b;
-                    double t = tmp[a]; tmp[a] = tmp[best]; tmp[best] = t; s += tmp[a];
-                }
-                s /= d.topk;
-            } else {
-                s = -1;
-                for (int v = 0; v < N; ++v) {
-                    if (fabs(s - sv[v]) < p->fold_agree * fabs(sv[v])) s = 0.5 * (s + sv[v]);
-                    else if (sv[v] > s) s = sv[v];
-                }
-            }
-            score[k] = s;
+            score[k] = score_pixel_d(kf, p, &d, i, j, plane4[4 * k], plane4[4 * k + 1], plane4[4 * k + 2], NULL,
+                                     view_scores ? view_scores + k * N : NULL);
         }
     release(&d);
     return 0;
+}
+
+/* The two halves of the score error, at n interior pixels pix[2 q] = (i, j):
+ * uv32 (n N T 2 floats) gets the canonical FP32 sample coordinates of every
+ * tap and view (the arithmetic of score_plane_f), uv64 (nullable, same
+ * shape) the exact ones (warp_tap_d). */
+EXPORT int or_warp_taps(const or_keyframe* kf, const or_params* p, const float* plane4, int64_t n, const int32_t* pix,
+                        float* uv32, double* uv64) {
+    derived_t d;
+    if (derive(kf, p, &d)) return -1;
+    int W = d.W, N = d.N, T = d.T;
+    int bad = 0;
+#pragma omp parallel for schedule(static) reduction(| : bad)
+    for (int64_t q = 0; q < n; ++q) {
+        int i = pix[2 * q], j = pix[2 * q + 1];
+        if (!interior(&d, i, j)) { bad = 1; continue; }
+        size_t k = (size_t)i * W + j;
+        float px = plane4[4 * k], py = plane4[4 * k + 1], pz = plane4[4 * k + 2];
+        for (int t = 0; t < T; ++t) {
+            int ii = i + d.k1[t], jj = j + d.k2[t];
+            float bx, by, bz, sb[3], as;
+            bearing_f(&d, ii, jj, &bx, &by, &bz);
+            tap_dir_f(bx, by, bz, px, py, pz, sb, &as);
+            for (int v = 0; v < N; ++v) {
+                float x, y, z;
+                view_dir_f(d.M[v], d.c[v], sb, as, &x, &y, &z);
+                size_t o = 2 * (((size_t)q * N + v) * T + t);
+                project_f(&d, x, y, z, &uv32[o], &uv32[o + 1]);
+                if (uv64) warp_tap_d(kf, &d, ii, jj, v, px, py, pz, &uv64[o], &uv64[o + 1]);
+            }
+        }
+    }
+    release(&d);
+    return bad ? -1 : 0;
+}
+
+/* FP64 score (score_pixel_d) at n interior pixels pix[2 q] = (i, j) with the
+ * samples taken at given coordinates uv32 (n N T 2, or_warp_taps' layout):
+ * the NCC arithmetic alone, with the warp's FP32 rounding factored out.
+ * score (n) and view_scores (n N, nullable). */
+EXPORT int or_score_at_coords_f64(const or_keyframe* kf, const or_params* p, const float* plane4, int64_t n,
+                                  const int32_t* pix, const float* uv32, double* score, double* view_scores) {
+    derived_t d;
+    if (derive(kf, p, &d)) return -1;
+    int W = d.W, N = d.N, T = d.T;
+    int bad = 0;
+#pragma omp parallel for schedule(static) reduction(| : bad)
+    for (int64_t q = 0; q < n; ++q) {
+        int i = pix[2 * q], j = pix[2 * q + 1];
+        if (!interior(&d, i, j)) { bad = 1; continue; }
+        size_t k = (size_t)i * W + j;
+        score[q] = score_pixel_d(kf, p, &d, i, j, plane4[4 * k], plane4[4 * k + 1], plane4[4 * k + 2],
+                                 uv32 + 2 * (size_t)q * N * T, view_scores ? view_scores + (size_t)q * N : NULL);
+    }
+    release(&d);
+    return bad ? -1 : 0;
 }
 
 /* ------------------------------------------------------------------ */

@@ -42,6 +42,9 @@ def lib():
         L.or_init.argtypes = [P(OrKeyframe), P(Params), _f32p, _f32p, _f32p]
         L.or_score_field.argtypes = [P(OrKeyframe), P(Params), _f32p, _f32p, _f32p]
         L.or_score_field_f64.argtypes = [P(OrKeyframe), P(Params), _f32p, _f64p, _f64p]
+        L.or_warp_taps.argtypes = [P(OrKeyframe), P(Params), _f32p, ctypes.c_int64, _i32p, _f32p, _f64p]
+        L.or_score_at_coords_f64.argtypes = [P(OrKeyframe), P(Params), _f32p, ctypes.c_int64, _i32p, _f32p, _f64p,
+                                             _f64p]
         L.or_propagate.argtypes = [P(OrKeyframe), P(Params), _f32p, _f32p, _f32p, ctypes.c_int, ctypes.c_int, _i64p]
         L.or_propagate_rows.argtypes = [P(OrKeyframe), P(Params), _f32p, _f32p, _f32p, ctypes.c_int, ctypes.c_int,
                                         ctypes.c_int, ctypes.c_int, _i64p]
@@ -75,7 +78,7 @@ def lib():
                                       _f64p, _f32p, _f32p, _f32p]
         for name in ("or_init", "or_score_field", "or_score_field_f64", "or_propagate", "or_propagate_rows",
                      "or_finalize", "or_densify", "or_clean", "or_clean_ex", "or_prune", "or_eval_math", "or_refine_normals",
-                     "or_tables", "or_derive_poses"):
+                     "or_tables", "or_derive_poses", "or_warp_taps", "or_score_at_coords_f64"):
             getattr(L, name).restype = ctypes.c_int
         _lib = L
     return _lib
@@ -144,6 +147,38 @@ def score_field_f64(kf: Keyframe, p: Params, plane4, views=False):
     _chk(lib().or_score_field_f64(ctypes.byref(kf.c), ctypes.byref(p), _p(plane4, _f32p), _p(score, _f64p),
                                   _p(vs, _f64p) if views else None), "score_field_f64")
     return (score, vs) if views else score
+
+
+def _taps(p: Params):
+    return len(range(-p.patch_half, p.patch_half + 1, p.patch_stride)) ** 2
+
+
+def warp_taps(kf: Keyframe, p: Params, plane4, pix, exact=True):
+    """FP32 sample coordinates (n, N, T, 2) of every tap and view at interior
+    pixels pix (n, 2) = (i, j), and (exact) their FP64 counterparts."""
+    plane4 = np.ascontiguousarray(plane4, dtype=np.float32)
+    pix = np.ascontiguousarray(pix, dtype=np.int32).reshape(-1, 2)
+    n, T = pix.shape[0], _taps(p)
+    uv32 = np.zeros((n, kf.N, T, 2), np.float32)
+    uv64 = np.zeros((n, kf.N, T, 2), np.float64) if exact else None
+    _chk(lib().or_warp_taps(ctypes.byref(kf.c), ctypes.byref(p), _p(plane4, _f32p), n, _p(pix, _i32p),
+                            _p(uv32, _f32p), _p(uv64, _f64p)), "warp_taps")
+    return uv32, uv64
+
+
+def score_at_coords_f64(kf: Keyframe, p: Params, plane4, pix, uv32):
+    """FP64 score and per-view scores at pixels pix with the samples taken at
+    the given coordinates uv32 (warp_taps' layout)."""
+    plane4 = np.ascontiguousarray(plane4, dtype=np.float32)
+    pix = np.ascontiguousarray(pix, dtype=np.int32).reshape(-1, 2)
+    uv32 = np.ascontiguousarray(uv32, dtype=np.float32)
+    n = pix.shape[0]
+    assert uv32.shape == (n, kf.N, _taps(p), 2)
+    s = np.zeros(n, np.float64)
+    sv = np.zeros((n, kf.N), np.float64)
+    _chk(lib().or_score_at_coords_f64(ctypes.byref(kf.c), ctypes.byref(p), _p(plane4, _f32p), n, _p(pix, _i32p),
+                                      _p(uv32, _f32p), _p(s, _f64p), _p(sv, _f64p)), "score_at_coords_f64")
+    return s, sv
 
 
 def propagate(kf: Keyframe, p: Params, plane4, depth, score, it_begin, it_end):

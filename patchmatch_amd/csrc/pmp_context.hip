@@ -295,7 +295,7 @@ void pmp_default_params(pmp_params* p) {
     p->refine_depth_decay = 0.3f;
     p->refine_normal_decay = 0.8f;
     p->min_patch_sd = 0.0f;
-    p->median_rel_thr = 0.05f;
+    p->median_rel_thr = 0.05;
 }
 
 int pmp_synchronize(pmp_ctx* ctx) {

@@ -28,7 +28,7 @@ int check_params(pmp_ctx* ctx, const pmp_params* p) {
     if (p->patch_half < 3 || p->patch_stride < 1 || p->patch_half > kMaxHalf)
         return set_error(ctx, PMP_EINVAL, "patch_half must be in [3, 10] (propagation reads 3 pixels away)");
     if (p->n_refine < 0 || p->n_refine > kMaxRefine) return set_error(ctx, PMP_EINVAL, "n_refine must be in [0, 8]");
-    if (p->agg_mode < 0 || p->agg_mode > 2 || p->edge_mode < 0 || p->edge_mode > 2 || p->coverage_mode < 0 ||
+    if (p->agg_mode < 0 || p->agg_mode > 3 || p->edge_mode < 0 || p->edge_mode > 2 || p->coverage_mode < 0 ||
         p->coverage_mode > 1 || p->init_mode < 0 || p->init_mode > 1)
         return set_error(ctx, PMP_EINVAL, "unknown mode value");
     if (!(p->sigma_color > 0.0f) || !(p->sigma_space > 0.0f))
@@ -66,6 +66,7 @@ int build_args(pmp_ctx* ctx, const pmp_params* p, uint64_t ref_id, int N, const 
     if (k > a.N) k = a.N;
     a.topk = k;
     a.inv_k = (float)(1.0 / (double)k);
+    for (int v = 0; v < kMaxViews; ++v) a.rn[v] = (float)(1.0 / (double)(v + 1));
     Tables tb;
     rc = get_tables(ctx, a.W, a.H, p->sigma_color, tb);
     if (rc) return rc;

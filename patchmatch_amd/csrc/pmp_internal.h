@@ -32,6 +32,7 @@ struct KfArgs {
     float ku, hu, kv, hv;                 // u = lon*ku + hu, v = asin*kv + hv
     float umax, vmax;                     // sample clamps: W (wrap) or W-1, and H-1
     float var_floor, fold_agree, inv_k;
+    float rn[kMaxViews];                  // 1/(v+1), rounded once (pooled moments of PMP_AGG_REF_FOLD)
     float depth_penalty, plane_penalty;
     float dr[kMaxRefine], nr[kMaxRefine];
     float min_depth, max_depth, inv_lo, inv_hi;

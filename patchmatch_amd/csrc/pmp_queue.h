@@ -34,7 +34,8 @@ struct SlotDesc {
     float* tmp0;              // median scratch
     float* tmp1;
     const uint8_t* mask;      // nullable
-    float min_var, inv_T, rel_thr, min_score;
+    double rel_thr;           // PostProcess median test (a double, depthmap.cc:1190)
+    float min_var, inv_T, min_score;
     int postprocess;
     int pass0;                // red-black pass of this slot at launch pass 0
     int nbx_init;             // 64-pixel tiles per row

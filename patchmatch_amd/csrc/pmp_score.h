@@ -14,6 +14,9 @@
 //   * weights Wc[|dI|] * Ws[tap] from tables;
 //   * shifted moments (reference taps centred on their weighted mean,
 //     neighbour samples shifted by the view's first-tap sample);
+//   * PMP_AGG_REF_FOLD pools the moments across views as the reference does
+//     (its sums are zeroed once, :57-62, outside the view loop :101), from
+//     the per-view shifted moments (pool_view);
 //   * column-pair packed uint8 neighbours (2 bytes per texel): one 32-bit
 //     load per tap and view.
 #pragma once
@@ -186,47 +189,51 @@ PMP_DF float in_vgpr(float s) {
     return v;
 }
 
-template <int NV>
-PMP_DF float aggregate(const KfArgs& a, float (&sv)[NV]) {
-    if (a.agg_mode == PMP_AGG_MAX) {
-        float s = -1.0f;
-#pragma unroll
-        for (int v = 0; v < NV; ++v) s = (sv[v] > s) ? sv[v] : s;
-        return s;
-    }
-    if (a.agg_mode == PMP_AGG_TOPK_MEAN) {
-        // exchange sort: position p ends with the max of positions p..NV-1,
-        // so positions 0..k-1 hold the k largest in descending order
-        float sum = 0.0f;
-#pragma unroll
-        for (int p = 0; p < NV; ++p) {
-#pragma unroll
-            for (int b = p + 1; b < NV; ++b) {
-                float hi = (sv[b] > sv[p]) ? sv[b] : sv[p];
-                float lo = (sv[b] > sv[p]) ? sv[p] : sv[b];
-                sv[p] = hi;
-                sv[b] = lo;
-            }
-            if (p < a.topk) sum = (p == 0) ? sv[0] : sum + sv[p];
-        }
-        return sum * a.inv_k;
-    }
-    float s = -1.0f;
-#pragma unroll
-    for (int v = 0; v < NV; ++v) {
-        float s2 = sv[v];
-        if (abs_(s - s2) < a.fold_agree * abs_(s2)) s = 0.5f * (s + s2);
-        else s = (s2 > s) ? s2 : s;
-    }
-    return s;
+// The fold of PatchMatchCuda.cu:171 with the agreement test written without
+// the division (identical at s2 = 0).
+PMP_DF float fold(float s, float s2, float agree) {
+    if (abs_(s - s2) < agree * abs_(s2)) return 0.5f * (s + s2);
+    return (s2 > s) ? s2 : s;
 }
 
-// Aggregation of N (run-time) per-view scores kept in LDS at sv[v * ld]
-// (v < N): the value of aggregate<NV> over the same scores (REF_FOLD and MAX
-// are the same sequential folds; for TOPK_MEAN each score is inserted into a
-// descending list of the topk largest, which are the first topk positions of
-// aggregate's exchange sort, and summed in the same order).  Per-view scores
-// are finite (ncc clamps to [-1, 1]).
+// Moments pooled across views (PMP_AGG_REF_FOLD).  The reference zeroes its
+// six sums once (PatchMatchCuda.cu:57-62) and the view loop (:101-157) keeps
+// adding to them, so view v's NCC (:161-169) is that of the taps of views
+// 0..v together: the reference side repeats (its mean and variance stay those
+// of one view), the pooled covariance is the mean of the per-view
+// covariances, and the pooled variance of y is the mean of the per-view
+// variances plus the spread of the per-view means.  C and V sum covariances
+// and variances, mu and q are Welford's running mean and sum of squared
+// deviations of the view means, s the fold.  Zero sums and s = -1 start it.
+struct Pool {
+    float C, V, mu, q, s;
+};
+
+// Adds view v (shifted moments acc0..2 about its shift) and folds the pooled
+// NCC of views 0..v into st.s.
+PMP_DF void pool_view(const KfArgs& a, const RefStat& rs, Pool& st, float rn, float acc0, float acc1, float acc2,
+                      float shift) {
+    const float my = acc0 * rs.inv;
+    const float vy = fma_(-my, my, acc1 * rs.inv);
+    const float cov = fma_(-rs.mxc, my, acc2 * rs.inv);
+    const float m = shift + my;
+    st.C = st.C + cov;
+    st.V = st.V + vy;
+    const float dm = m - st.mu;
+    st.mu = fma_(dm, rn, st.mu);
+    st.q = fma_(dm, m - st.mu, st.q);
+    const float cp = st.C * rn;
+    const float vp = (st.V + st.q) * rn;
+    float s2 = -1.0f;
+    if (!(rs.vx < a.var_floor || vp < a.var_floor)) s2 = clampf(cp * rsqrt_pos(rs.vx * vp), -1.0f, 1.0f);
+    st.s = fold(st.s, s2, a.fold_agree);
+}
+
+// Aggregation of N (run-time) independent per-view scores kept in LDS at
+// sv[v * ld] (v < N): FOLD_INDEP folds them in view order, MAX takes the
+// largest, TOPK_MEAN inserts each into a descending list of the topk largest
+// and sums that list in order.  Per-view scores are finite (ncc clamps to
+// [-1, 1]).
 PMP_DF float aggregate_rt(const KfArgs& a, const float* sv, int ld) {
     const int N = a.N;
     if (a.agg_mode == PMP_AGG_TOPK_MEAN) {
@@ -259,11 +266,7 @@ PMP_DF float aggregate_rt(const KfArgs& a, const float* sv, int ld) {
         }
         return s;
     }
-    for (int v = 0; v < N; ++v) {
-        const float s2 = sv[v * ld];
-        if (abs_(s - s2) < a.fold_agree * abs_(s2)) s = 0.5f * (s + s2);
-        else s = (s2 > s) ? s2 : s;
-    }
+    for (int v = 0; v < N; ++v) s = fold(s, sv[v * ld], a.fold_agree);
     return s;
 }
 
@@ -340,6 +343,12 @@ struct LdsTaps {
     PMP_DF float refpx(uint32_t roff, int jj) const { return (float)sref[roff + (uint32_t)jj]; }
 };
 
+// Shifted moments of one view: a0 = sum w yd, a1 = sum w yd^2,
+// a2 = sum w xc yd with yd = y - sh (sh = the first tap's sample).
+struct ViewMom {
+    float a0, a1, a2, sh;
+};
+
 // Moments of views [v0, v0+G) (G <= 3 live views keep the per-view
 // transforms in scalar registers; v0 is uniform).  Per view the taps are
 // accumulated in tap order, exactly as in a single pass over all views.
@@ -348,7 +357,7 @@ struct LdsTaps {
 // the loop starts at the second tap.
 template <int G, bool kZero, class Taps>
 PMP_DF void view_group(const KfArgs& a, const Taps& tp, const float* sWc, const RefStat& rs, int i, int j, float px,
-                       float py, float pz, int v0, float (&out)[G]) {
+                       float py, float pz, int v0, ViewMom (&out)[G]) {
     const uint32_t* base[G];
 #pragma unroll
     for (int g = 0; g < G; ++g) base[g] = a.vq[v0 + g];
@@ -434,57 +443,58 @@ PMP_DF void view_group(const KfArgs& a, const Taps& tp, const float* sWc, const 
         for (int g = 0; g < G; ++g) accumulate(g, blend<kZero>(pd[g]), w, wxc);
     }
 #pragma unroll
-    for (int g = 0; g < G; ++g) out[g] = ncc(rs, a.var_floor, acc0[g], acc1[g], acc2[g]);
+    for (int g = 0; g < G; ++g) out[g] = ViewMom{acc0[g], acc1[g], acc2[g], shift[g]};
 }
 
-template <int NV, int V0, bool kZero, class Taps>
-PMP_DF void view_groups(const KfArgs& a, const Taps& tp, const float* sWc, const RefStat& rs, int i, int j, float px,
-                        float py, float pz, float (&sv)[NV]) {
-    if constexpr (V0 < NV) {
-        constexpr int G = (NV - V0) < 3 ? (NV - V0) : 3;
-        float out[G];
-        view_group<G, kZero>(a, tp, sWc, rs, i, j, px, py, pz, V0, out);
+// Hands the moments of views [v0, v0+G) on: PMP_AGG_REF_FOLD pools them into
+// the running state kept in LDS rows 0..4 of sv (fresh at v0 = 0); the other
+// aggregations park each view's own NCC in row v.
+template <int G>
+PMP_DF void emit_views(const KfArgs& a, const RefStat& rs, int v0, const ViewMom (&m)[G], float* sv, int ld) {
+    if (a.agg_mode == PMP_AGG_REF_FOLD) {
+        Pool st{0.0f, 0.0f, 0.0f, 0.0f, -1.0f};
+        if (v0 > 0) st = Pool{sv[0], sv[ld], sv[2 * ld], sv[3 * ld], sv[4 * ld]};
 #pragma unroll
-        for (int g = 0; g < G; ++g) sv[V0 + g] = out[g];
-        view_groups<NV, V0 + G, kZero>(a, tp, sWc, rs, i, j, px, py, pz, sv);
+        for (int g = 0; g < G; ++g) pool_view(a, rs, st, a.rn[v0 + g], m[g].a0, m[g].a1, m[g].a2, m[g].sh);
+        sv[0] = st.C;
+        sv[ld] = st.V;
+        sv[2 * ld] = st.mu;
+        sv[3 * ld] = st.q;
+        sv[4 * ld] = st.s;
+    } else {
+#pragma unroll
+        for (int g = 0; g < G; ++g) sv[(v0 + g) * ld] = ncc(rs, a.var_floor, m[g].a0, m[g].a1, m[g].a2);
     }
 }
 
 // Score of plane (px, py, pz) at pixel (i, j); kZero selects PMP_EDGE_ZERO
-// sampling.  NV > 0: compile-time view count (sv unused); NV == 0: a.N views
-// (uniform), in groups of 3, 2 or 1, with the per-view scores parked in LDS
-// at sv[v * ld] until they are aggregated.
+// sampling.  a.N views (uniform) in groups of 3, 2 or 1; between groups the
+// per-view scores (or the pooled state) are parked in LDS at sv[r * ld], r <
+// kMaxViews, until they are aggregated.  NV must be 0 (run-time view count).
 // tp: the source of the per-tap inputs (GlobalTaps, or LdsTaps for a tile).
 template <int NV, bool kZero, class Taps>
 PMP_DF float score_plane_t(const KfArgs& a, const Taps& tp, const float* sWc, const RefStat& rs, int i, int j,
                            float px, float py, float pz, float* sv, int ld) {
-    if constexpr (NV > 0) {
-        float s[NV];
-        view_groups<NV, 0, kZero>(a, tp, sWc, rs, i, j, px, py, pz, s);
-        return aggregate<NV>(a, s);
-    } else {
-        const int N = a.N;
-        for (int v0 = 0; v0 < N; v0 += 3) {
-            const int G = (N - v0) < 3 ? (N - v0) : 3;
-            if (G == 3) {
-                float out[3];
-                view_group<3, kZero>(a, tp, sWc, rs, i, j, px, py, pz, v0, out);
-                sv[v0 * ld] = out[0];
-                sv[(v0 + 1) * ld] = out[1];
-                sv[(v0 + 2) * ld] = out[2];
-            } else if (G == 2) {
-                float out[2];
-                view_group<2, kZero>(a, tp, sWc, rs, i, j, px, py, pz, v0, out);
-                sv[v0 * ld] = out[0];
-                sv[(v0 + 1) * ld] = out[1];
-            } else {
-                float out[1];
-                view_group<1, kZero>(a, tp, sWc, rs, i, j, px, py, pz, v0, out);
-                sv[v0 * ld] = out[0];
-            }
+    static_assert(NV == 0, "the view count is a run-time value");
+    const int N = a.N;
+    for (int v0 = 0; v0 < N; v0 += 3) {
+        const int G = (N - v0) < 3 ? (N - v0) : 3;
+        if (G == 3) {
+            ViewMom m[3];
+            view_group<3, kZero>(a, tp, sWc, rs, i, j, px, py, pz, v0, m);
+            emit_views<3>(a, rs, v0, m, sv, ld);
+        } else if (G == 2) {
+            ViewMom m[2];
+            view_group<2, kZero>(a, tp, sWc, rs, i, j, px, py, pz, v0, m);
+            emit_views<2>(a, rs, v0, m, sv, ld);
+        } else {
+            ViewMom m[1];
+            view_group<1, kZero>(a, tp, sWc, rs, i, j, px, py, pz, v0, m);
+            emit_views<1>(a, rs, v0, m, sv, ld);
         }
-        return aggregate_rt(a, sv, ld);
     }
+    if (a.agg_mode == PMP_AGG_REF_FOLD) return sv[4 * ld];
+    return aggregate_rt(a, sv, ld);
 }
 
 template <int NV, bool kZero>

@@ -325,12 +325,13 @@ __device__ __forceinline__ void median3_pixel(const float* __restrict__ src, flo
     dst[(size_t)i * W + j] = v[4];
 }
 
-// PostProcess test (depthmap.cc:1189-1196)
+// PostProcess test (depthmap.cc:1189-1196): the float ratio |d - m| / d against
+// the double threshold (fabs of a float is float there, 0.05 a double literal)
 __device__ __forceinline__ void postfilter_pixel(const float* __restrict__ med, float* __restrict__ depth,
-                                                 float* __restrict__ score, size_t k, float rel_thr,
+                                                 float* __restrict__ score, size_t k, double rel_thr,
                                                  float min_depth, float max_depth, float min_score) {
     const float d = depth[k], m = med[k], sc = score[k];
-    if (d == 0.0f || (double)(__builtin_fabsf(d - m) / d) > (double)rel_thr || d < min_depth || d > max_depth ||
+    if (d == 0.0f || (double)(__builtin_fabsf(d - m) / d) > rel_thr || d < min_depth || d > max_depth ||
         sc <= min_score) {
         depth[k] = 0.0f;
         score[k] = -1.0f;

@@ -10,11 +10,15 @@ Defaults are the reference's operating values:
   (``depthmap_computation_module.cc:223``), depth range [1e-5, ...]
   (``depthmap_computation_module.cc:123-131``), 2 iterations
   (``config.cc`` default ``Dense.Iterations``), median test 0.05
-  (``depthmap.cc:1190``).
+  (``depthmap.cc:1190``, a double: the float ratio is compared with the
+  double literal there).
 """
 import ctypes
 
-AGG_REF_FOLD, AGG_MAX, AGG_TOPK_MEAN = 0, 1, 2
+# REF_FOLD: the fold of PatchMatchCuda.cu:161-171 over moments pooled across
+# views (zeroed once at :57-62, outside the view loop :101); FOLD_INDEP: the
+# same fold over independent per-view NCCs (no counterpart in the reference)
+AGG_REF_FOLD, AGG_MAX, AGG_TOPK_MEAN, AGG_FOLD_INDEP = 0, 1, 2, 3
 EDGE_WRAP, EDGE_CLAMP, EDGE_ZERO = 0, 1, 2
 COVER_REF_GRID, COVER_FULL = 0, 1
 INIT_REF_UNIFORM, INIT_INVDEPTH_HEMI = 0, 1
@@ -36,7 +40,7 @@ class Params(ctypes.Structure):
         ("depth_penalty", ctypes.c_float), ("plane_penalty", ctypes.c_float),
         ("refine_depth_range", ctypes.c_float), ("refine_normal_range", ctypes.c_float),
         ("refine_depth_decay", ctypes.c_float), ("refine_normal_decay", ctypes.c_float),
-        ("min_patch_sd", ctypes.c_float), ("median_rel_thr", ctypes.c_float),
+        ("min_patch_sd", ctypes.c_float), ("median_rel_thr", ctypes.c_double),
     ]
 
     def copy(self, **kw):

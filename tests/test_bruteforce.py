@@ -13,7 +13,7 @@ import pytest
 
 from oracle import or_py
 from patchmatch_amd import synth
-from patchmatch_amd.params import AGG_MAX, COVER_FULL, default_params
+from patchmatch_amd.params import AGG_MAX, AGG_REF_FOLD, COVER_FULL, default_params
 
 ADJ = [(0, 1), (1, 0), (0, -1), (-1, 0), (0, 3), (3, 0), (0, -3), (-3, 0)]
 
@@ -21,7 +21,10 @@ ADJ = [(0, 1), (1, 0), (0, -1), (-1, 0), (0, 3), (3, 0), (0, -3), (-3, 0)]
 class PyScorer:
     """FP64 plane score, restated: taps (k1 outer, k2 inner), bilateral weight,
     equirect warp with longitude wrap, bilinear sample, one-pass moments, NCC,
-    max over views (CPU aggregation, depthmap.cc:1110-1117)."""
+    then either the max over views (AGG_MAX, the CPU aggregation
+    depthmap.cc:1110-1117) or the device fold (AGG_REF_FOLD,
+    PatchMatchCuda.cu:161-171) with the six sums carried from view to view
+    (zeroed once, :57-62)."""
 
     def __init__(self, kf, p):
         self.kf, self.p = kf, p
@@ -62,19 +65,28 @@ class PyScorer:
         Xc = b / s[:, None]
         x = self.ref[ii, jj]
         w = np.exp(-(x - self.ref[i, j]) ** 2 * self.cf - (self.k1 ** 2 + self.k2 ** 2) * self.xf)
+        pooled = self.p.agg_mode == AGG_REF_FOLD
         best = -1.0
+        sums = np.zeros(6)
         for M, c, img in self.views:
             X = Xc @ M.T + c
             r = np.maximum(np.linalg.norm(X, axis=1), 1e-6)
             u = self.W * (0.5 + np.arctan2(X[:, 0], X[:, 2]) / (2 * np.pi))
             v = self.H * (0.5 + np.arcsin(np.clip(X[:, 1] / r, -1, 1)) / np.pi)
             y = self.sample(img, u, v)
-            sw = w.sum()
-            mx, my = (w * x).sum() / sw, (w * y).sum() / sw
-            vx = (w * x * x).sum() / sw - mx * mx
-            vy = (w * y * y).sum() / sw - my * my
-            ncc = -1.0 if (vx < self.p.var_floor or vy < self.p.var_floor) else ((w * x * y).sum() / sw - mx * my) / np.sqrt(vx * vy)
-            best = max(best, min(1.0, max(-1.0, ncc)))
+            view = np.array([w.sum(), (w * x).sum(), (w * y).sum(), (w * x * x).sum(), (w * y * y).sum(),
+                             (w * x * y).sum()])
+            sums = sums + view if pooled else view
+            sw, sx, sy, sxx, syy, sxy = sums
+            mx, my = sx / sw, sy / sw
+            vx = sxx / sw - mx * mx
+            vy = syy / sw - my * my
+            ncc = -1.0 if (vx < self.p.var_floor or vy < self.p.var_floor) else (sxy / sw - mx * my) / np.sqrt(vx * vy)
+            ncc = min(1.0, max(-1.0, ncc))
+            if pooled:
+                best = 0.5 * (best + ncc) if abs((best - ncc) / ncc if ncc != 0 else np.inf) < 0.25 else max(best, ncc)
+            else:
+                best = max(best, ncc)
         return best
 
 
@@ -126,10 +138,11 @@ def py_red_black(kf, okf, p, plane, depth, score, it):
     return plane, depth, score
 
 
-def test_red_black_pass_bruteforce():
+@pytest.mark.parametrize("agg", [AGG_MAX, AGG_REF_FOLD])
+def test_red_black_pass_bruteforce(agg):
     kf = synth.make_keyframe(5, 64, 32, 2, baseline=(0.15, 0.3))
     okf = or_py.Keyframe.from_synth(kf)
-    p = default_params(patch_half=3, patch_stride=2, agg_mode=AGG_MAX, coverage_mode=COVER_FULL,
+    p = default_params(patch_half=3, patch_stride=2, agg_mode=agg, coverage_mode=COVER_FULL,
                        init_mode=1, seed=21)
     plane, depth, score = or_py.init(okf, p)
     oP, oD, oS, _ = or_py.propagate(okf, p, plane, depth, score, 0, 1)

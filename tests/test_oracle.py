@@ -11,8 +11,8 @@ import pytest
 
 from oracle import or_py
 from patchmatch_amd import synth
-from patchmatch_amd.params import (AGG_MAX, AGG_REF_FOLD, AGG_TOPK_MEAN, COVER_FULL, EDGE_WRAP, config0_params,
-                                   config1_params, default_params)
+from patchmatch_amd.params import (AGG_FOLD_INDEP, AGG_MAX, AGG_REF_FOLD, AGG_TOPK_MEAN, COVER_FULL, EDGE_WRAP,
+                                   config0_params, config1_params, default_params)
 
 from conftest import gt_planes, random_planes
 
@@ -255,8 +255,10 @@ def test_ncc_variance_floor():
     assert (s[np.isfinite(s)] == -1).all()
 
 
-def test_view_aggregation_truth_tables():
-    fold = default_params(agg_mode=AGG_REF_FOLD)
+@pytest.mark.parametrize("agg", [AGG_REF_FOLD, AGG_FOLD_INDEP])
+def test_view_aggregation_truth_tables(agg):
+    # the fold itself (PatchMatchCuda.cu:171); REF_FOLD applies it to pooled per-view scores
+    fold = default_params(agg_mode=agg)
     # s=-1, s2=0.8 -> |(-1-0.8)| < 0.25*0.8 false -> max -> 0.8; then s2=0.7: |0.1| < 0.175 -> avg 0.75
     assert np.isclose(or_py.aggregate(fold, [0.8, 0.7]), 0.75)
     assert np.isclose(or_py.aggregate(fold, [0.8, 0.3]), 0.8)          # no agreement -> max
@@ -269,15 +271,87 @@ def test_view_aggregation_truth_tables():
     assert np.isclose(or_py.aggregate(default_params(agg_mode=AGG_TOPK_MEAN, topk=9), [0.5, 0.1]), 0.3)
 
 
+def _weighted_ref_stats(ref, half, stride, sigma_c, sigma_s):
+    """Bilateral-weighted mean and variance of every pixel's reference taps, in
+    float64 from the weight's definition (PatchMatchCuda.cu:149): taps at
+    offsets -half..half by stride, w = exp(-(x - x_centre)^2 / (2 sc^2) -
+    |k|^2 / (2 ss^2)).  NaN where the patch leaves the image."""
+    H, W = ref.shape
+    x0 = ref.astype(np.float64)
+    sw = np.zeros((H, W)); swx = np.zeros((H, W)); swxx = np.zeros((H, W))
+    ks = range(-half, half + 1, stride)
+    for k1 in ks:
+        for k2 in ks:
+            x = np.roll(np.roll(x0, -k1, 0), -k2, 1)
+            w = np.exp(-(x - x0) ** 2 / (2 * sigma_c ** 2) - (k1 * k1 + k2 * k2) / (2 * sigma_s ** 2))
+            sw += w; swx += w * x; swxx += w * x * x
+    mx = swx / sw
+    vx = swxx / sw - mx * mx
+    edge = np.zeros((H, W), bool)
+    edge[:half + 1] = edge[H - half - 1:] = True
+    edge[:, :half + 1] = edge[:, W - half - 1:] = True
+    mx[edge] = np.nan
+    vx[edge] = np.nan
+    return mx, vx
+
+
+def test_pooled_fold_two_view_closed_form():
+    """REF_FOLD pools the moment sums across views, as PatchMatchCuda.cu does
+    (zeroed once at :57-62, outside the view loop :101; NCC of the running
+    sums at :161-169).  Identity poses and views that are exact affine maps of
+    the reference, y0 = x + 30 and y1 = 2x + 10, in integers: view 0's NCC is 1;
+    view 1's is that of the two views' samples together,
+        cov = (vx + 2 vx)/2,  var_y = (vx + 4 vx)/2 + ((m0 - m1)/2)^2,
+    m0 - m1 = (mx + 30) - (2 mx + 10) = 20 - mx, with mx, vx the weighted mean
+    and variance of the reference taps.  Independent per-view NCCs
+    (FOLD_INDEP, MAX) are 1 and 1.  The FP32 oracle (per-view shifted moments
+    pooled by running sums and Welford's update) and the FP64 scorer (raw
+    sums that run on across views) must both give the closed form."""
+    base = _textured(96, 192, seed=3)
+    ref = (base.astype(np.int32) * 100 // 255).astype(np.uint8)
+    views = [(ref.astype(np.int32) + 30).astype(np.uint8), (2 * ref.astype(np.int32) + 10).astype(np.uint8)]
+    pose = synth.yaw_pose(0.0, [0, 0, 0])
+    kf = or_py.Keyframe(ref, views, pose, np.stack([pose, pose]))
+    H, W = ref.shape
+    B = synth.bearings(W, H)
+    p4 = np.zeros((H, W, 4), np.float32)
+    p4[..., :3] = B * 0.5                  # fronto-parallel, depth 2 (any plane: identity poses)
+    half, stride = 3, 1
+    p = default_params(patch_half=half, patch_stride=stride, agg_mode=AGG_REF_FOLD)
+    mx, vx = _weighted_ref_stats(ref, half, stride, p.sigma_color, p.sigma_space)
+    e1 = 1.5 * vx / np.sqrt(vx * (2.5 * vx + ((20.0 - mx) / 2.0) ** 2))
+    fold = np.where(np.abs(1.0 - e1) < 0.25 * e1, 0.5 * (1.0 + e1), np.maximum(1.0, e1))
+    m = np.isfinite(vx) & (vx > 1.0)
+    assert m.mean() > 0.8
+    for name, fn in (("fp32", or_py.score_field), ("fp64", or_py.score_field_f64)):
+        s, sv = fn(kf, p, p4, views=True)
+        d0 = np.abs(sv[..., 0][m] - 1.0)
+        d1 = np.abs(sv[..., 1][m] - e1[m])
+        ds = np.abs(s[m] - fold[m])
+        print(f"{name}: view0 max {d0.max():.1e}, pooled view1 max {d1.max():.1e}, folded max {ds.max():.1e}")
+        # identity-pose warps carry ~1e-4 px of FP32 rounding (FP64: none)
+        assert np.mean(d1 <= 1e-5) >= 0.99 and d1.max() < 1e-3, name
+        assert np.mean(d0 <= 1e-5) >= 0.99 and ds.max() < 1e-3, name
+    # the pooled value is not the independent one: it drops below 1 by the
+    # spread of the view means, here by more than 0.01 on most pixels
+    assert np.mean(1.0 - e1[m] > 0.01) > 0.5
+    for agg in (AGG_FOLD_INDEP, AGG_MAX):
+        s, sv = or_py.score_field(kf, p.copy(agg_mode=agg), p4, views=True)
+        assert np.mean(np.abs(sv[..., 1][m] - 1.0) <= 1e-5) >= 0.99
+        assert np.mean(np.abs(s[m] - 1.0) <= 1e-5) >= 0.99
+
+
 # ------------------------------------------------------------------ FP32 vs FP64 (fixed hypothesis field)
-@pytest.mark.parametrize("cfg", ["c0", "small6"])
+@pytest.mark.parametrize("cfg", ["c0", "c0_topk", "small6"])
 def test_fp32_score_vs_fp64_oracle(cfg, kf_c0, kf_small6):
     """The NCC of a fixed hypothesis field against the FP64 scorer (exact
     bearings, two-step transform, libm functions, single-exp weight, raw
-    moments).  Per view: |d| <= 1e-5 on >= 99 % of interior pixels and
-    <= 2e-3 everywhere; aggregated (best-k mean): <= 1e-5 on >= 99 %."""
-    kf = kf_c0 if cfg == "c0" else kf_small6
-    p = config0_params(agg_mode=AGG_TOPK_MEAN, topk=2) if cfg == "c0" else config1_params()
+    moments; for c0's REF_FOLD the raw sums run on across views, as in the
+    reference).  Per view: |d| <= 1e-5 on >= 99 % of interior pixels and
+    <= 2e-3 everywhere; aggregated: <= 1e-5 on >= 99 %."""
+    kf = kf_small6 if cfg == "small6" else kf_c0
+    p = {"c0": config0_params(), "c0_topk": config0_params(agg_mode=AGG_TOPK_MEAN, topk=2),
+         "small6": config1_params()}[cfg]
     ok = or_py.Keyframe.from_synth(kf)
     planes = random_planes(kf, 3, jitter=0.05)
     s32, v32 = or_py.score_field(ok, p, planes, views=True)
@@ -371,6 +445,92 @@ def test_finalize_window_and_postprocess(kf_c0):
     kept = out["depth"] > 0
     assert (out["score"][~kept] == -1).all()
     assert kept.mean() > 0.3
+
+
+def _postprocess(depth, score, **kw):
+    """or_finalize with the same maps before and after the passes, no mask and
+    no low-texture gate: pixels inside the copy-back window take the final
+    maps, the others the initial ones, both (depth, score) here, so the
+    output is PostProcess (depthmap.cc:1169-1208) of (depth, score) everywhere."""
+    H, W = depth.shape
+    img = np.zeros((H, W), np.uint8)
+    pose = synth.yaw_pose(0.0, [0, 0, 0])
+    kf = or_py.Keyframe(img, [img], pose, pose[None])
+    p = default_params(patch_half=3, patch_stride=1, **kw)
+    pl = np.zeros((H, W, 4), np.float32)
+    maps = (pl, np.ascontiguousarray(depth, np.float32), np.ascontiguousarray(score, np.float32))
+    od, _, osc, _ = or_py.finalize(kf, p, None, maps, maps)
+    return od, osc
+
+
+def test_postprocess_median_is_textbook_median_twice():
+    """The kept depth is the 3x3 median with replicated border (cv::medianBlur,
+    ksize 3) applied twice (depthmap.cc:1172-1173), against scipy's
+    median_filter(size=3, mode='nearest') twice; the rejection clauses are
+    switched off (threshold 1e300, depth range and score floor out of reach),
+    so only d = 0 rejects.  One pass, or a zero border, would differ."""
+    from scipy.ndimage import median_filter
+    rng = np.random.default_rng(4)
+    _, d = synth.render(synth.make_scene(8), synth.yaw_pose(0.3, [0.1, 0, 0]), 96, 48)
+    d = d.astype(np.float32) * rng.choice([1.0, 1.0, 0.7, 1.4], d.shape).astype(np.float32)
+    d[rng.random(d.shape) < 0.05] = 0.0
+    sc = rng.uniform(-0.5, 1.0, d.shape).astype(np.float32)
+    od, osc = _postprocess(d, sc, median_rel_thr=1e300, min_depth=-1e30, max_depth=1e30, min_score=-2.0)
+    m2 = median_filter(median_filter(d, size=3, mode="nearest"), size=3, mode="nearest")
+    keep = d != 0
+    assert np.array_equal(od[keep].view(np.uint32), m2[keep].view(np.uint32))
+    assert (od[~keep] == 0).all() and (osc[~keep] == -1).all()
+    assert np.array_equal(osc[keep], sc[keep])
+    m1 = median_filter(d, size=3, mode="nearest")
+    assert np.mean(m1[keep] != m2[keep]) > 0.05
+    mz = median_filter(median_filter(d, size=3, mode="constant"), size=3, mode="constant")
+    border = np.zeros(d.shape, bool)
+    border[[0, -1], :] = True
+    border[:, [0, -1]] = True
+    assert np.count_nonzero((mz != m2) & border & keep) > 10
+
+
+# (m: the map's value around the pixel, d: the pixel's depth, sc: its score,
+#  parameter overrides, kept?)  PostProcess (depthmap.cc:1189-1196) rejects when
+#  d == 0, or fabs(d - m) / d > 0.05 with the float quotient against the double
+#  literal, or d < min_depth, or d > max_depth, or sc <= min_score.
+_F = np.float32
+POSTPROCESS_TABLE = [
+    ("plain", 19.0, 19.0, 0.5, {}, True),
+    ("d = 0", 19.0, 0.0, 0.5, {}, False),
+    # (20 - 19) / 20 rounds to 0.05f = 0.0500000007 > 0.05 (double): rejected
+    ("ratio = 0.05f above", 19.0, 20.0, 0.5, {}, False),
+    ("ratio = 0.05f below", 21.0, 20.0, 0.5, {}, False),
+    # (20 - 19.000002) / 20 rounds below 0.05: kept (takes the median 19.000002)
+    ("ratio just below 0.05", float(np.nextafter(_F(19.0), _F(20.0))), 20.0, 0.5, {}, True),
+    ("ratio 0.06", 18.8, 20.0, 0.5, {}, False),
+    ("d = min_depth", 19.0, 19.0, 0.5, {"min_depth": 19.0}, True),
+    ("d < min_depth", 19.0, 19.0, 0.5, {"min_depth": float(np.nextafter(_F(19.0), _F(20.0)))}, False),
+    ("d = max_depth", 19.0, 19.0, 0.5, {"max_depth": 19.0}, True),
+    ("d > max_depth", 19.0, 19.0, 0.5, {"max_depth": float(np.nextafter(_F(19.0), _F(0.0)))}, False),
+    ("score = min_score", 19.0, 19.0, float(_F(0.1)), {}, False),
+    ("score just above min_score", 19.0, 19.0, float(np.nextafter(_F(0.1), _F(1.0))), {}, True),
+    ("score -1", 19.0, 19.0, -1.0, {}, False),
+    ("score NaN (never scored, 0xFF memset)", 19.0, 19.0, float("nan"), {}, True),
+]
+
+
+@pytest.mark.parametrize("case", POSTPROCESS_TABLE, ids=[c[0] for c in POSTPROCESS_TABLE])
+def test_postprocess_truth_table(case):
+    """One clause at a time: a pixel of depth d and score sc in a 32x32 map of
+    value m (so both medians give m there); kept pixels take the median and
+    keep their score, rejected ones become (0, -1)."""
+    _, m, d, sc, kw, kept = case
+    depth = np.full((32, 32), m, np.float32)
+    score = np.full((32, 32), 0.5, np.float32)
+    depth[16, 16] = d
+    score[16, 16] = sc
+    kw = dict(dict(min_depth=1e-5, max_depth=100.0, min_score=0.1), **kw)
+    od, osc = _postprocess(depth, score, **kw)
+    if kept:
+        assert od[16, 16] == _F(m) and (osc[16, 16] == _F(sc) or (np.isnan(sc) and np.isnan(osc[16, 16])))
+    else:
+        assert od[16, 16] == 0.0 and osc[16, 16] == -1.0
 
 
 # ------------------------------------------------------------------ committed fixture
