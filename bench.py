@@ -39,12 +39,16 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-FP32_PEAK_TFLOPS = 157.3       # MI355X published FP32 peak
+FP32_PEAK_TFLOPS = 157.3            # MI355X published FP32 vector peak (counts packed FP32 FMA)
+FP32_NONPACKED_PEAK_TFLOPS = 78.6   # the same without packed FP32, which this build does not use
 HBM_PEAK_GBS = 8000.0
 # L2-to-memory bytes per k_propagate_q launch on the 1920x960 + 6-view
 # workload, from rocprofv3 FETCH_SIZE and WRITE_SIZE scaled by a copy of known
 # size (profiles/k_propagate_q_c1_1920x960_6views.md); None for other workloads.
 TRAFFIC_BYTES_PER_LAUNCH_C1 = int((370_600 * 2.00 + 42_982 * 1.00) * 1024)
+TRAFFIC_SOURCE = ("profile-derived, not measured in this run: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE of "
+                  "k_propagate_q on this workload, scaled by a streaming copy of known size "
+                  "(profiles/k_propagate_q_c1_1920x960_6views.md)")
 
 
 def ops_per_eval(T: int, N: int) -> int:
@@ -203,6 +207,10 @@ def main(argv=None):
     evals_per_launch = evals / max(launches, 1)
     opc = ops_per_eval(T, N)
     achieved_tflops = evals_per_launch * opc / (avg_launch_ms * 1e-3) / 1e12
+    # the reference scores all 8 candidates and every refinement of every
+    # pixel of the pass's colour (no pre-rejection): half the interior pixels
+    ref_evals_per_launch = P_int * (8 + p.n_refine) / 2
+    ref_equiv_tflops = ref_evals_per_launch * opc / (avg_launch_ms * 1e-3) / 1e12
     ms_per_step = wall * 1e3 / args.steps
     value = args.steps / wall
     mpx_iter = P_int * p.iterations / 1e6
@@ -234,11 +242,24 @@ def main(argv=None):
         "evals_per_step": int(evals / args.steps),
         "candidates_per_step": int(cands / args.steps),
         "accuracy_vs_ground_truth": acc,
-        "roofline": {"bound": "mfma", "kernel": "k_propagate_q", "achieved": round(achieved_tflops, 3),
+        "roofline": {"bound": "valu", "kernel": "k_propagate_q", "achieved": round(achieved_tflops, 3),
                      "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved_tflops / FP32_PEAK_TFLOPS, 4),
+                     "peak_nonpacked": FP32_NONPACKED_PEAK_TFLOPS,
+                     "frac_nonpacked": round(achieved_tflops / FP32_NONPACKED_PEAK_TFLOPS, 4),
                      "traffic": TRAFFIC_BYTES_PER_LAUNCH_C1 if (args.width, args.height, N) == (1920, 960, 6) else None,
-                     "traffic_unit": "bytes per launch (L2 misses, rocprofv3)", "ops_per_eval": opc, "evals_per_launch": int(evals_per_launch),
-                     "avg_launch_ms": round(avg_launch_ms, 4), "launches": launches},
+                     "traffic_unit": "bytes per launch (L2 misses)", "traffic_source": TRAFFIC_SOURCE,
+                     "ops_model": "T (19 + 52 N) + 25 N + 10 FP32 ops per hypothesis evaluation "
+                                  "(FMA 2, add/mul/compare 1, each elementary function 1); executed evaluations "
+                                  "only (pre-rejected candidates are not scored)",
+                     "ops_per_eval": opc, "evals_per_launch": int(evals_per_launch),
+                     "avg_launch_ms": round(avg_launch_ms, 4), "launches": launches,
+                     "reference_equivalent": {
+                         "note": "every candidate and refinement of every pixel scored, as the reference does "
+                                 "(P (8 + n_refine) / 2 evaluations per pass); not what this kernel executes",
+                         "evals_per_launch": int(ref_evals_per_launch),
+                         "achieved": round(ref_equiv_tflops, 3),
+                         "frac": round(ref_equiv_tflops / FP32_PEAK_TFLOPS, 4),
+                         "frac_nonpacked": round(ref_equiv_tflops / FP32_NONPACKED_PEAK_TFLOPS, 4)}},
     }
 
     if args.cpu_seconds > 0:

@@ -140,6 +140,10 @@ int pmp_resize(pmp_ctx* ctx, const uint8_t* src, int32_t width, int32_t height, 
                uint8_t* dst, int32_t out_width, int32_t out_height, int32_t on_device);
 /* Number of images and device bytes held by the pool. */
 int pmp_image_pool_stats(const pmp_ctx* ctx, int64_t* count, int64_t* bytes);
+/* Width and height of a pooled image (PMP_ENOTFOUND if absent): lets a caller
+ * size the buffers of pmp_image_get and of the masks it passes.  No
+ * counterpart in the reference, whose images are cv::Mat with their size. */
+int pmp_image_size(const pmp_ctx* ctx, uint64_t id, int32_t* width, int32_t* height);
 
 /* ---- one keyframe (DepthmapEstimator, depthmap.cc:541-806) -------------- */
 

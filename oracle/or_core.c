@@ -27,7 +27,7 @@
  *   - shifted moments (same covariance and variances as the raw one-pass
  *     moments of :150-169, without their FP32 cancellation): reference taps
  *     centred on their weighted mean, neighbour samples shifted by the
- *     view's first-tap sample;
+ *     view's sample at the centre tap, the tap walk starting there;
  *   - explicit neighbour bilinear with longitude wrap / latitude clamp;
  *   - the reference fold (REF_FOLD) over moments pooled across views, as
  *     the device accumulates them (:57-62 zeroed once, outside the view loop
@@ -60,6 +60,7 @@
 /* ------------------------------------------------------------------ */
 typedef struct {
     int W, H, N, T, half, stride, n1;
+    int tc;   /* centre tap (the middle of the row-major tap grid): the shift's tap */
     int k1[OR_MAX_TAPS], k2[OR_MAX_TAPS];
     float Ws[OR_MAX_TAPS];
     float Wc[256];
@@ -134,6 +135,10 @@ static int derive(const or_keyframe* kf, const or_params* p, derived_t* d) {
             d->k1[T] = a; d->k2[T] = b; ++T;
         }
     d->T = T;
+    {
+        int n = 2 * d->half / d->stride + 1;   /* taps per row */
+        d->tc = (n / 2) * n + n / 2;
+    }
     double xf = 1.0 / (2.0 * (double)p->sigma_space * (double)p->sigma_space);
     double cf = 1.0 / (2.0 * (double)p->sigma_color * (double)p->sigma_color);
     for (int t = 0; t < T; ++t) d->Ws[t] = (float)exp(-(double)(d->k1[t] * d->k1[t] + d->k2[t] * d->k2[t]) * xf);
@@ -290,7 +295,7 @@ static inline refstat_t ref_stats(const derived_t* d, const uint8_t* ref, int i,
 }
 
 /* NCC of one view from the shifted moments: cov = E[xc y] - E[xc] E[y],
- * var_y = E[y^2] - E[y]^2 (y shifted by the view's first-tap sample) */
+ * var_y = E[y^2] - E[y]^2 (y shifted by the view's centre-tap sample) */
 static inline float ncc_f(const refstat_t* rs, float floor_, float Swy, float Swyy, float Swxy) {
     float my = Swy * rs->inv;
     float vy = fmaf(-my, my, Swyy * rs->inv);
@@ -374,7 +379,11 @@ static float score_plane_f(const derived_t* d, const or_params* p, const or_keyf
     for (int v = 0; v < N; ++v) { acc0[v] = 0.0f; acc1[v] = 0.0f; acc2[v] = 0.0f; }
     float shift[OR_MAX_VIEWS];
     const uint8_t* ref = kf->ref;
-    for (int t = 0; t < d->T; ++t) {
+    /* taps from the centre tap tc to the last, then from the first to tc - 1;
+     * the centre tap's sample is each view's shift (it lies near the weighted
+     * mean far more often than a corner tap: smaller cancellation) */
+    for (int q = 0; q < d->T; ++q) {
+        int t = (q + d->tc) % d->T;
         int ii = i + d->k1[t], jj = j + d->k2[t];
         float bx, by, bz, sb[3], as;
         bearing_f(d, ii, jj, &bx, &by, &bz);
@@ -388,7 +397,7 @@ static float score_plane_f(const derived_t* d, const or_params* p, const or_keyf
             float u, vv;
             project_f(d, x, y, z, &u, &vv);
             float val = sample_f(kf->views[v], W, H, p->edge_mode, u, vv);
-            if (t == 0) shift[v] = val;
+            if (q == 0) shift[v] = val;
             float yd = val - shift[v];
             float tw = w * yd;
             acc0[v] = acc0[v] + tw;

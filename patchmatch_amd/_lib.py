@@ -50,7 +50,7 @@ EXPORTS = [
     "pmp_server_submit", "pmp_server_wait", "pmp_server_fetch", "pmp_server_stats", "pmp_server_pause",
     "pmp_preinit_landmarks", "pmp_memorize_result", "pmp_preinit_warp", "pmp_forget_result",
     "pmp_write_dense_msgpack", "pmp_write_ply", "pmp_image_put_thumbnail", "pmp_resize", "pmp_image_get",
-    "pmp_score_view", "pmp_depth_interval",
+    "pmp_score_view", "pmp_depth_interval", "pmp_image_size",
 ]
 
 PMP_MAX_VIEWS = 10
@@ -151,6 +151,7 @@ def load():
         "pmp_image_put": ([_vp, u64, _u8p, i32, i32, i64, i32], c_int),
         "pmp_image_drop": ([_vp, u64], c_int),
         "pmp_image_pool_stats": ([_vp, _i64p, _i64p], c_int),
+        "pmp_image_size": ([_vp, u64, _i32p, _i32p], c_int),
         "pmp_set_views": ([_vp, u64, P(Pose), i32, _u64p, P(Pose)], c_int),
         "pmp_random_initialization": ([_vp, P(Params)], c_int),
         "pmp_score_plane_depth": ([_vp, P(Params)], c_int),
@@ -324,11 +325,29 @@ class Context:
         self._chk(self.L.pmp_resize(self.h, _p(img, _u8p), W, H, W * C, C, _p(out, _u8p), oW, oH, 0), "pmp_resize")
         return out
 
-    def image_get(self, image_id: int, shape):
-        """Copy a pooled grey image back to the host."""
-        out = np.zeros(shape, np.uint8)
+    def image_size(self, image_id: int):
+        """(height, width) of a pooled image."""
+        w, h = ctypes.c_int32(), ctypes.c_int32()
+        self._chk(self.L.pmp_image_size(self.h, image_id, ctypes.byref(w), ctypes.byref(h)), "pmp_image_size")
+        return h.value, w.value
+
+    def image_get(self, image_id: int, shape=None):
+        """Copy a pooled grey image back to the host (shape, if given, must be its (H, W))."""
+        hw = self.image_size(image_id)
+        if shape is not None and tuple(shape) != hw:
+            raise ValueError(f"image {image_id} is {hw[0]}x{hw[1]} (H x W), not {tuple(shape)}")
+        out = np.zeros(hw, np.uint8)
         self._chk(self.L.pmp_image_get(self.h, image_id, _p(out, _u8p)), "pmp_image_get")
         return out
+
+    def _mask(self, mask):
+        """A host mask as contiguous uint8 of the current keyframe's (H, W), or None."""
+        if mask is None:
+            return None
+        m = np.ascontiguousarray(mask, dtype=np.uint8)
+        if m.shape != (self.H, self.W):
+            raise ValueError(f"mask shape {m.shape} differs from the keyframe's {(self.H, self.W)}")
+        return m
 
     def image_drop(self, image_id: int):
         self._chk(self.L.pmp_image_drop(self.h, image_id), "pmp_image_drop")
@@ -344,6 +363,7 @@ class Context:
         poses = (Pose * n)(*[Pose.from12(p) for p in view_poses])
         self._chk(self.L.pmp_set_views(self.h, ref_id, ctypes.byref(Pose.from12(ref_pose)), n, ids, poses),
                   "pmp_set_views")
+        self.H, self.W = self.image_size(ref_id)
 
     def load_keyframe(self, ref, views, ref_pose, view_poses, base_id=0):
         """Upload reference + neighbours (ids base_id, base_id+1, ...) and select them."""
@@ -351,7 +371,6 @@ class Context:
         for k, v in enumerate(views):
             self.image_put(base_id + 1 + k, v)
         self.set_views(base_id, ref_pose, [base_id + 1 + k for k in range(len(views))], view_poses)
-        self.H, self.W = np.asarray(ref).shape
 
     # -- stages
     def random_initialization(self, p: Params):
@@ -364,16 +383,16 @@ class Context:
         self._chk(self.L.pmp_red_black_pass(self.h, ctypes.byref(p), it_begin, it_end), "pmp_red_black_pass")
 
     def finalize(self, p: Params, mask=None):
-        m = None if mask is None else np.ascontiguousarray(mask, dtype=np.uint8)
+        m = self._mask(mask)
         self._chk(self.L.pmp_finalize(self.h, ctypes.byref(p), _p(m, _u8p)), "pmp_finalize")
 
     def densify(self, p: Params, mask=None):
-        m = None if mask is None else np.ascontiguousarray(mask, dtype=np.uint8)
+        m = self._mask(mask)
         self._chk(self.L.pmp_densify(self.h, ctypes.byref(p), _p(m, _u8p)), "pmp_densify")
 
     def preinit_landmarks(self, p: Params, xyz, mask=None) -> int:
         xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
-        m = None if mask is None else np.ascontiguousarray(mask, dtype=np.uint8)
+        m = self._mask(mask)
         acc = ctypes.c_int64()
         self._chk(self.L.pmp_preinit_landmarks(self.h, ctypes.byref(p), len(xyz), _p(xyz, _f64p), _p(m, _u8p),
                                                ctypes.byref(acc)), "pmp_preinit_landmarks")
@@ -386,7 +405,7 @@ class Context:
         self._chk(self.L.pmp_forget_result(self.h), "pmp_forget_result")
 
     def preinit_warp(self, p: Params, mask=None) -> int:
-        m = None if mask is None else np.ascontiguousarray(mask, dtype=np.uint8)
+        m = self._mask(mask)
         acc = ctypes.c_int64()
         self._chk(self.L.pmp_preinit_warp(self.h, ctypes.byref(p), _p(m, _u8p), ctypes.byref(acc)),
                   "pmp_preinit_warp")
@@ -598,7 +617,8 @@ class Server:
             if a is None:
                 return None
             a = np.ascontiguousarray(a, dtype=np.uint8)
-            assert a.shape == (H, W)
+            if a.shape != (H, W):
+                raise ValueError(f"image shape {a.shape} differs from the request's {(H, W)}")
             keep.append(a)
             return a.ctypes.data_as(_u8p)
         r.ref_pixels = px(ref)

@@ -382,6 +382,15 @@ int pmp_image_drop(pmp_ctx* ctx, uint64_t id) {
     return PMP_OK;
 }
 
+int pmp_image_size(const pmp_ctx* ctx, uint64_t id, int32_t* width, int32_t* height) {
+    if (!ctx) return PMP_EINVAL;
+    auto it = ctx->images.find(id);
+    if (it == ctx->images.end()) return PMP_ENOTFOUND;
+    if (width) *width = it->second.W;
+    if (height) *height = it->second.H;
+    return PMP_OK;
+}
+
 int pmp_image_pool_stats(const pmp_ctx* ctx, int64_t* count, int64_t* bytes) {
     if (!ctx) return PMP_EINVAL;
     int64_t b = 0;

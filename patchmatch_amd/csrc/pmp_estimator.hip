@@ -58,6 +58,12 @@ int build_args(pmp_ctx* ctx, const pmp_params* p, uint64_t ref_id, int N, const 
                                    (2.0 * (double)p->sigma_space * (double)p->sigma_space));
         }
     a.T = T;
+    {
+        const int n = 2 * a.half / a.stride + 1;   // taps per row (and per column)
+        a.tc = (n / 2) * n + n / 2;
+        a.kc = -a.half + (n / 2) * a.stride;
+        a.klast = -a.half + (n - 1) * a.stride;
+    }
     if (a.W < 2 * a.half + 8 || a.H < 2 * a.half + 8) return set_error(ctx, PMP_EINVAL, "image too small for the patch");
     if ((size_t)a.W * a.H * 4 > 0x7FFFFFFFull) return set_error(ctx, PMP_EINVAL, "image too large (4*W*H must be < 2^31)");
     a.agg_mode = p->agg_mode; a.edge_mode = p->edge_mode; a.coverage = p->coverage_mode;

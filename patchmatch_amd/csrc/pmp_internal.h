@@ -20,6 +20,7 @@ constexpr int kMaxHalf = 10;   // largest patch half-width (patch_half)
 // Passed by value (kernarg segment, read with scalar loads).
 struct KfArgs {
     int W, H, N, half, stride, T;
+    int tc, kc, klast;                    // centre tap index and offset ((kc, kc), the shift tap), last tap offset
     int agg_mode, topk, edge_mode, coverage, init_mode, n_refine;
     const uint8_t* ref;                   // H*W
     const uint32_t* vq[kMaxViews];        // per view: H*(W+2) packed column-pair halfwords (k_pack)

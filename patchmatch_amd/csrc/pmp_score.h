@@ -13,7 +13,7 @@
 //   * longitude and latitude through one atan polynomial (lonlat);
 //   * weights Wc[|dI|] * Ws[tap] from tables;
 //   * shifted moments (reference taps centred on their weighted mean,
-//     neighbour samples shifted by the view's first-tap sample);
+//     neighbour samples shifted by the view's centre-tap sample);
 //   * PMP_AGG_REF_FOLD pools the moments across views as the reference does
 //     (its sums are zeroed once, :57-62, outside the view loop :101), from
 //     the per-view shifted moments (pool_view);
@@ -344,17 +344,20 @@ struct LdsTaps {
 };
 
 // Shifted moments of one view: a0 = sum w yd, a1 = sum w yd^2,
-// a2 = sum w xc yd with yd = y - sh (sh = the first tap's sample).
+// a2 = sum w xc yd with yd = y - sh (sh = the centre tap's sample).
 struct ViewMom {
     float a0, a1, a2, sh;
 };
 
 // Moments of views [v0, v0+G) (G <= 3 live views keep the per-view
-// transforms in scalar registers; v0 is uniform).  Per view the taps are
-// accumulated in tap order, exactly as in a single pass over all views.
-// The first tap's sample is the view's shift, so that tap adds exact zeros
-// to the three moments (yd = 0, w >= 0): it is sampled before the loop and
-// the loop starts at the second tap.
+// transforms in scalar registers; v0 is uniform).  The taps are walked from
+// the centre tap tc (offsets (kc, kc), the middle of the row-major tap grid)
+// to the last and on from the first to tc - 1, the same order for every
+// view.  The centre tap's sample is the view's shift: it lies near the
+// bilateral-weighted mean of the samples far more often than a corner tap,
+// which keeps the moments' cancellation small.  It adds exact zeros to the
+// three moments (yd = 0, w >= 0), so it is sampled before the loop and the
+// loop starts at the tap after it.
 template <int G, bool kZero, class Taps>
 PMP_DF void view_group(const KfArgs& a, const Taps& tp, const float* sWc, const RefStat& rs, int i, int j, float px,
                        float py, float pz, int v0, ViewMom (&out)[G]) {
@@ -364,8 +367,8 @@ PMP_DF void view_group(const KfArgs& a, const Taps& tp, const float* sWc, const 
     float acc0[G], acc1[G], acc2[G], shift[G];
     const float hu = in_vgpr(a.hu), hv = in_vgpr(a.hv);
     {
-        const float2 rt = tp.row(i - a.half);
-        const float2 ct = tp.col(j - a.half);
+        const float2 rt = tp.row(i + a.kc);
+        const float2 ct = tp.col(j + a.kc);
         float sbx, sby, sbz, as;
         tap_dir(rt.x * ct.x, -rt.y, rt.x * ct.y, px, py, pz, sbx, sby, sbz, as);
 #pragma unroll
@@ -376,20 +379,28 @@ PMP_DF void view_group(const KfArgs& a, const Taps& tp, const float* sWc, const 
             acc2[g] = 0.0f;
         }
     }
-    // Taps t = 1 .. T-1 in order, software-pipelined: view g's gather for
-    // tap t+1 is issued right after view g's tap t is blended, into the same
-    // registers, so each load has about one tap of work to arrive.
+    // The other T-1 taps in that order, software-pipelined: view g's gather
+    // for the next tap is issued right after view g's current tap is blended,
+    // into the same registers, so each load has about one tap of work to
+    // arrive.
     const int T = a.T;
-    // tap (k1, k2) and roff = tp.rowoff(i + k1) + j, the reference offset of its row
-    // (k1, k2 are the same in every lane; readfirstlane keeps them, and the
-    // row step, in scalar registers instead of per-lane copies)
+    // tap index ti at (k1, k2) and roff = tp.rowoff(i + k1) + j, the
+    // reference offset of its row (ti, k1, k2 are the same in every lane;
+    // readfirstlane keeps them, and the row steps, in scalar registers
+    // instead of per-lane copies); after the last tap the walk wraps to the
+    // first, klast - first_k rows up
     const int first_k = __builtin_amdgcn_readfirstlane(-a.half);
-    auto next = [&](int& k1, int& k2, uint32_t& roff) {
+    const uint32_t row_step = (uint32_t)a.stride * tp.rowstep();
+    const uint32_t wrap_step = 0u - (uint32_t)(a.klast + a.half) * tp.rowstep();
+    auto next = [&](int& ti, int& k1, int& k2, uint32_t& roff) {
         const int k2n = k2 + a.stride;
+        const int k1n = k1 + a.stride;
         const bool wrap = k2n > a.half;
+        const bool wrap_all = wrap && k1n > a.half;
         k2 = __builtin_amdgcn_readfirstlane(wrap ? first_k : k2n);
-        k1 = __builtin_amdgcn_readfirstlane(wrap ? k1 + a.stride : k1);
-        roff += wrap ? (uint32_t)a.stride * tp.rowstep() : 0u;
+        k1 = __builtin_amdgcn_readfirstlane(wrap ? (wrap_all ? first_k : k1n) : k1);
+        ti = __builtin_amdgcn_readfirstlane(ti + 1 == T ? 0 : ti + 1);
+        roff += wrap ? (wrap_all ? wrap_step : row_step) : 0u;
     };
     // direction and reference weights of tap t at (k1, k2)
     auto tap = [&](int k1, int k2, uint32_t roff, int t, float& sbx, float& sby, float& sbz, float& as, float& w,
@@ -409,14 +420,14 @@ PMP_DF void view_group(const KfArgs& a, const Taps& tp, const float* sWc, const 
         acc2[g] = fma_(wxc, yd, acc2[g]);
     };
     if (T > 1) {
-        int k1 = -a.half, k2 = -a.half;
-        uint32_t roff = tp.rowoff(i - a.half) + (uint32_t)j;
-        next(k1, k2, roff);
+        int ti = a.tc, k1 = a.kc, k2 = a.kc;
+        uint32_t roff = tp.rowoff(i + a.kc) + (uint32_t)j;
+        next(ti, k1, k2, roff);
         Pend<kZero> pd[G];
         float w, wxc;
         {
             float sbx, sby, sbz, as;
-            tap(k1, k2, roff, 1, sbx, sby, sbz, as, w, wxc);
+            tap(k1, k2, roff, ti, sbx, sby, sbz, as, w, wxc);
 #pragma unroll
             for (int g = 0; g < G; ++g) view_gather<kZero>(a, hu, hv, base[g], v0 + g, sbx, sby, sbz, as, pd[g]);
         }
@@ -426,9 +437,9 @@ PMP_DF void view_group(const KfArgs& a, const Taps& tp, const float* sWc, const 
         // at the top of the next.
 #pragma unroll 1
         for (int t = 1; t + 1 < T; ++t) {
-            next(k1, k2, roff);
+            next(ti, k1, k2, roff);
             float sbx, sby, sbz, as, nw, nwxc;
-            tap(k1, k2, roff, t + 1, sbx, sby, sbz, as, nw, nwxc);
+            tap(k1, k2, roff, ti, sbx, sby, sbz, as, nw, nwxc);
             __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
             for (int g = 0; g < G; ++g) {

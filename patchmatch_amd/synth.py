@@ -305,13 +305,12 @@ def make_keyframe(seed: int, W: int, H: int, n_views: int, baseline=(0.1, 0.3),
     prng = np.random.default_rng(s_photo)
     allposes = [ref_pose] + poses
     if W * H * len(allposes) >= 4_000_000 and workers > 1:
-        # noise-free renders in worker processes (spawned: no inherited GPU
-        # state); photometric noise afterwards, in order, so results do not
-        # depend on the worker count
+        # noise-free renders on worker threads (numpy releases the GIL in the
+        # large array operations; no child process outlives the call);
+        # photometric noise afterwards, in order, so results do not depend on
+        # the worker count
         import concurrent.futures as cf
-        import multiprocessing as mp
-        with cf.ProcessPoolExecutor(max_workers=min(workers, len(allposes)),
-                                    mp_context=mp.get_context("spawn")) as ex:
+        with cf.ThreadPoolExecutor(max_workers=min(workers, len(allposes))) as ex:
             rendered = list(ex.map(render, [scene] * len(allposes), allposes, [W] * len(allposes),
                                    [H] * len(allposes)))
     else:
