@@ -41,7 +41,8 @@ enum {
     PMP_EDEVICE = -3,    /* HIP runtime error */
     PMP_ENOTFOUND = -4,  /* unknown image id */
     PMP_ESTATE = -5,     /* call out of order (e.g. no views set) */
-    PMP_EBUSY = -6       /* not finished within the time limit */
+    PMP_EBUSY = -6,      /* not finished within the time limit */
+    PMP_ECANCELED = -7   /* a served request ended by pmp_server_cancel or pmp_server_reset */
 };
 
 /* PMP_AGG_REF_FOLD: the fold of PatchMatchCuda.cu:161-171 with the moment sums
@@ -345,12 +346,14 @@ typedef struct pmp_request_info {
 typedef struct pmp_server_info {
     int64_t submitted, completed, failed, steps;
     int64_t pool_images, pool_bytes, maps_bytes, uploads, evictions;
-    int32_t in_flight, queued, max_in_flight, reserved;
+    int32_t in_flight, queued, max_in_flight, cancelled;
 } pmp_server_info;
 
 int pmp_server_create(pmp_server** out, const pmp_server_opts* opts);
-/* Finishes every submitted request, then releases the device. */
+/* Finishes every submitted request (queued ones too, unless paused), then
+ * releases the device (request_terminate, :596-615). */
 void pmp_server_destroy(pmp_server* srv);
+/* The reason of the last failing pmp_server_* call made on this thread. */
 const char* pmp_server_last_error(pmp_server* srv);
 /* Queues a request with its own params (edge_mode must be the server's).
  * Missing images are uploaded (and packed) before it returns; waits while
@@ -367,8 +370,22 @@ int pmp_server_fetch(pmp_server* srv, uint64_t ticket, float* plane4, float* dep
                      pmp_request_info* info);
 int pmp_server_stats(pmp_server* srv, pmp_server_info* out);
 /* paused != 0: admit no queued request until resumed (requests in flight
- * continue); submissions are still accepted. */
+ * continue); submissions are still accepted.  The densify thread's
+ * request_pause / resume (depthmap_computation_module.cc:553-589). */
 int pmp_server_pause(pmp_server* srv, int32_t paused);
+/* Ends one request: a queued one at once, one in flight after the steps
+ * already issued for it have run (its slot then takes the next request).
+ * The ticket stays until fetched; wait returns and fetch gives
+ * PMP_ECANCELED.  A request already finished keeps its result.  No
+ * counterpart in the reference, whose densify thread has no per-keyframe
+ * cancellation. */
+int pmp_server_cancel(pmp_server* srv, uint64_t ticket);
+/* request_reset (depthmap_computation_module.cc:503-546): cancels every
+ * queued and in-flight request (their tickets fetch PMP_ECANCELED), waits
+ * until none is in flight, and empties the image pool (the reference clears
+ * its keyframe queue and depth-map database).  Blocks, as there; submissions
+ * wait meanwhile.  The server stays usable. */
+int pmp_server_reset(pmp_server* srv);
 
 /* ---- geometric consistency (DepthmapCleaner, depthmap.cc:1320-1377) ----- */
 

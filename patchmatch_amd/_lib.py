@@ -17,7 +17,7 @@ LIB_PATH = os.path.join(HERE, "libpmp.so")
 
 PMP_MAPS_WORK, PMP_MAPS_INIT, PMP_MAPS_FINAL = 0, 1, 2
 STATUS = {0: "PMP_OK", -1: "PMP_EINVAL", -2: "PMP_ENOMEM", -3: "PMP_EDEVICE", -4: "PMP_ENOTFOUND", -5: "PMP_ESTATE",
-          -6: "PMP_EBUSY"}
+          -6: "PMP_EBUSY", -7: "PMP_ECANCELED"}
 
 _f32p = ctypes.POINTER(ctypes.c_float)
 _f64p = ctypes.POINTER(ctypes.c_double)
@@ -48,6 +48,7 @@ EXPORTS = [
     "pmp_batch_densify", "pmp_batch_get_maps", "pmp_batch_set_config", "pmp_batch_get_config",
     "pmp_batch_autotune", "pmp_server_create", "pmp_server_destroy", "pmp_server_last_error",
     "pmp_server_submit", "pmp_server_wait", "pmp_server_fetch", "pmp_server_stats", "pmp_server_pause",
+    "pmp_server_cancel", "pmp_server_reset",
     "pmp_preinit_landmarks", "pmp_memorize_result", "pmp_preinit_warp", "pmp_forget_result",
     "pmp_write_dense_msgpack", "pmp_write_ply", "pmp_image_put_thumbnail", "pmp_resize", "pmp_image_get",
     "pmp_score_view", "pmp_depth_interval", "pmp_image_size",
@@ -111,7 +112,7 @@ class RequestInfo(ctypes.Structure):
 class ServerInfo(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int64) for n in ("submitted", "completed", "failed", "steps", "pool_images",
                                               "pool_bytes", "maps_bytes", "uploads", "evictions")] + \
-               [(n, ctypes.c_int32) for n in ("in_flight", "queued", "max_in_flight", "reserved")]
+               [(n, ctypes.c_int32) for n in ("in_flight", "queued", "max_in_flight", "cancelled")]
 
 
 class CameraInfo(ctypes.Structure):
@@ -189,6 +190,8 @@ def load():
         "pmp_server_fetch": ([_vp, u64, _f32p, _f32p, _f32p, _i32p, P(RequestInfo)], c_int),
         "pmp_server_stats": ([_vp, P(ServerInfo)], c_int),
         "pmp_server_pause": ([_vp, i32], c_int),
+        "pmp_server_cancel": ([_vp, u64], c_int),
+        "pmp_server_reset": ([_vp], c_int),
         "pmp_preinit_landmarks": ([_vp, P(Params), i64, _f64p, _u8p, _i64p], c_int),
         "pmp_memorize_result": ([_vp], c_int),
         "pmp_forget_result": ([_vp], c_int),
@@ -657,6 +660,14 @@ class Server:
 
     def pause(self, paused: bool = True):
         self._chk(self.L.pmp_server_pause(self.h, 1 if paused else 0), "pmp_server_pause")
+
+    def cancel(self, ticket: int):
+        """End one request (fetch then raises PMP_ECANCELED unless it had finished)."""
+        self._chk(self.L.pmp_server_cancel(self.h, ticket), "pmp_server_cancel")
+
+    def reset(self):
+        """Cancel every queued and in-flight request and empty the image pool (blocks)."""
+        self._chk(self.L.pmp_server_reset(self.h), "pmp_server_reset")
 
     def stats(self) -> dict:
         s = ServerInfo()

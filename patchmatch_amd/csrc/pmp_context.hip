@@ -22,17 +22,21 @@ int check_hip(pmp_ctx* ctx, hipError_t e, const char* what) {
 // Bearing tables (PatchMatchCuda.cu:191-198), double precision rounded once:
 // rowtab[i] = (cos lat_i, sin lat_i), lat_i = -pi (i/H - 1/2);
 // coltab[j] = (sin lon_j, cos lon_j), lon_j = 2 pi (j/W - 1/2).
+float2 table_row(int i, int H) {
+    const double lat = -M_PI * ((double)i / (double)H - 0.5);
+    return make_float2((float)cos(lat), (float)sin(lat));
+}
+
+float2 table_col(int j, int W) {
+    const double lon = 2.0 * M_PI * ((double)j / (double)W - 0.5);
+    return make_float2((float)sin(lon), (float)cos(lon));
+}
+
 void host_tables(int W, int H, std::vector<float2>& rowtab, std::vector<float2>& coltab) {
     rowtab.resize(H);
     coltab.resize(W);
-    for (int i = 0; i < H; ++i) {
-        double lat = -M_PI * ((double)i / (double)H - 0.5);
-        rowtab[i] = make_float2((float)cos(lat), (float)sin(lat));
-    }
-    for (int j = 0; j < W; ++j) {
-        double lon = 2.0 * M_PI * ((double)j / (double)W - 0.5);
-        coltab[j] = make_float2((float)sin(lon), (float)cos(lon));
-    }
+    for (int i = 0; i < H; ++i) rowtab[i] = table_row(i, H);
+    for (int j = 0; j < W; ++j) coltab[j] = table_col(j, W);
 }
 
 // M = R_n R_ref^T, c = t_n - M t_ref in double, rounded once.  Equivalent to

@@ -181,6 +181,11 @@ int get_tables(pmp_ctx* ctx, int W, int H, float sigma_color, Tables& t);
 // first use, on `stream`).
 int ensure_quads(pmp_ctx* ctx, Image& im, int wrap, hipStream_t stream);
 void host_tables(int W, int H, std::vector<float2>& rowtab, std::vector<float2>& coltab);
+// One entry of the bearing tables: row i of H (cos lat, sin lat), column j of
+// W (sin lon, cos lon), in double rounded once (ComputeBearingsEquirect_,
+// PatchMatchCuda.cu:191-198).
+float2 table_row(int i, int H);
+float2 table_col(int j, int W);
 void derive_poses(const double* ref_pose, int N, const double* view_poses, float (*M)[9], float (*c)[3]);
 int check_params(pmp_ctx* ctx, const pmp_params* p);
 void queue_free(pmp_ctx* ctx);

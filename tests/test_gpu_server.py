@@ -228,3 +228,46 @@ def test_server_rejects_bad_requests(gpu_ctx):
                                                           (kf.H, kf.W), ref=kf.ref, views=kf.views)
         with pytest.raises(PmpError, match="ENOTFOUND"):
             srv.wait(987654, 0)
+
+
+def test_server_cancel_and_reset(gpu_ctx):
+    """pmp_server_cancel ends a queued request at once and one in flight once
+    its issued steps have run; pmp_server_reset cancels everything queued or
+    in flight and empties the pool.  The requests that ran are bit-identical
+    to the plain path, and the server keeps serving after both."""
+    kfs = _keyframes()
+    want = _plain(gpu_ctx, kfs)
+    with Server(0, max_slots=2) as srv:
+        srv.pause(True)
+        t = [_submit(srv, kfs, k) for k in range(len(kfs))]
+        srv.cancel(t[3])          # queued
+        srv.pause(False)
+        srv.cancel(t[0])          # admitted first: in flight (or already finished)
+        done = {}
+        for k in range(len(kfs)):
+            assert srv.wait(t[k], 60000)
+            try:
+                done[k] = srv.fetch(t[k])
+            except PmpError as e:
+                assert "ECANCELED" in str(e) and k in (0, 3), (k, str(e))
+        assert 3 not in done and len(done) >= len(kfs) - 2
+        for k, got in done.items():
+            _check(got, want[k], f"request {k} next to cancelled ones")
+        assert srv.stats()["cancelled"] >= 1
+        # reset with requests queued and in flight (their images are pooled)
+        t2 = [_submit(srv, kfs, k, with_pixels=False) for k in (1, 2, 4, 5, 6)]
+        srv.reset()
+        st = srv.stats()
+        print(f"[server reset] {st}", flush=True)
+        assert st["in_flight"] == 0 and st["queued"] == 0 and st["pool_images"] == 0 and st["pool_bytes"] == 0
+        for k, tk in zip((1, 2, 4, 5, 6), t2):
+            assert srv.wait(tk, 0)
+            try:
+                _check(srv.fetch(tk), want[k], f"request {k} finished before the reset")
+            except PmpError as e:
+                assert "ECANCELED" in str(e)
+        with pytest.raises(PmpError, match="ENOTFOUND"):
+            _submit(srv, kfs, 1, with_pixels=False)
+        tk = _submit(srv, kfs, 1)
+        assert srv.wait(tk, 60000)
+        _check(srv.fetch(tk), want[1], "after the reset")
