@@ -20,7 +20,11 @@ forms of the metric (patchmatch_amd/workloads.py):
              a 500-keyframe trajectory (125 keyframes per resolution class,
              up to 5760x2880, views U{2..10}), served (continuous batching,
              LRU image pool) vs the plain per-request path, p50/p99 latency
-             from arrival to maps on the host;
+             from arrival to maps on the host; each form's capacity (all
+             requests at once) and a rate at half the served capacity; every
+             request of a stream checked bit for bit against plain under an
+             HBM budget that evicts images;
+  "served_1920x960": the same at 1920x960 with 6 neighbours only;
   "pipeline": 30 keyframes of one trajectory at the plain size, densified
              against their 4-8 nearest neighbours, cleaned (5-map window,
              depth within 1 %, round trip within 1 px, normals within 10
@@ -49,6 +53,11 @@ TRAFFIC_BYTES_PER_LAUNCH_C1 = int((370_600 * 2.00 + 42_982 * 1.00) * 1024)
 TRAFFIC_SOURCE = ("profile-derived, not measured in this run: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE of "
                   "k_propagate_q on this workload, scaled by a streaming copy of known size "
                   "(profiles/k_propagate_q_c1_1920x960_6views.md)")
+# VALU wave-instructions per k_propagate_q launch on the 1920x960 + 6-view
+# workload (rocprofv3 --pmc SQ_INSTS_VALU, same profile): the issue side of
+# the roofline, against one wave64 VALU instruction per CU per clock.
+VALU_WAVE_INSTS_PER_LAUNCH_C1 = 3.289e9
+CUS, CLOCK_HZ = 256, 2.4e9
 
 
 def ops_per_eval(T: int, N: int) -> int:
@@ -138,7 +147,7 @@ def main(argv=None):
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline sample length (0: skip)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: min(16, available cores)")
     ap.add_argument("--batched", type=int, default=64, help="keyframes of the batched workload (0: skip)")
-    ap.add_argument("--served-requests", type=int, default=40, help="requests per served rate (0: skip)")
+    ap.add_argument("--served-requests", type=int, default=24, help="requests per served rate (0: skip)")
     ap.add_argument("--rates", type=str, default="5,15,30", help="served arrival rates (requests/s)")
     ap.add_argument("--pipeline", type=int, default=30,
                     help="keyframes of the densify -> clean -> fuse trajectory at the plain size (0: skip)")
@@ -253,6 +262,17 @@ def main(argv=None):
                                   "only (pre-rejected candidates are not scored)",
                      "ops_per_eval": opc, "evals_per_launch": int(evals_per_launch),
                      "avg_launch_ms": round(avg_launch_ms, 4), "launches": launches,
+                     "valu_issue": ({
+                         "source": "SQ_INSTS_VALU per launch from the same profile (not measured in this run), "
+                                   "over this run's launch time",
+                         "lane_instructions_per_sample": round(VALU_WAVE_INSTS_PER_LAUNCH_C1 * 64 /
+                                                               (evals_per_launch * T * N), 1),
+                         "wave_instructions_per_s": round(VALU_WAVE_INSTS_PER_LAUNCH_C1 / (avg_launch_ms * 1e-3), -8),
+                         "peak_wave_instructions_per_s": CUS * CLOCK_HZ,
+                         "peak_note": "one wave64 VALU instruction per CU per clock at 2.4 GHz (the non-packed "
+                                      "FP32 FMA rate); the profile's issue rate per SIMD is the measured ceiling",
+                         "frac": round(VALU_WAVE_INSTS_PER_LAUNCH_C1 / (avg_launch_ms * 1e-3) / (CUS * CLOCK_HZ), 3)}
+                         if (args.width, args.height, N) == (1920, 960, 6) else None),
                      "reference_equivalent": {
                          "note": "every candidate and refinement of every pixel scored, as the reference does "
                                  "(P (8 + n_refine) / 2 evaluations per pass); not what this kernel executes",
@@ -305,6 +325,20 @@ def main(argv=None):
             rates = [float(r) for r in args.rates.split(",") if r]
             result["served"] = workloads.run_served(st, p, rates, args.served_requests, args.seed + 4)
             print(f"[bench] served: {result['served']}", file=sys.stderr, flush=True)
+            # every request of the stream against plain, under a budget that evicts images
+            specs = st.draw(np.random.default_rng(args.seed + 6), 16)
+            big = 5760 * 2880
+            _, one = workloads.image_bytes(st, specs)
+            result["served"]["verification"] = workloads.verify_served(
+                st, p, specs, max_slots=2, hbm_budget=2 * 84 * big + 2 * one + (64 << 20), window=2)
+            print(f"[bench] served verification: {result['served']['verification']}", file=sys.stderr, flush=True)
+            del st
+            # the plain workload's shape alone (1920x960, 6 neighbours) below capacity: the latency target's case
+            st = workloads.Stream(args.seed + 7, [(1920, 960, 1.0)], per_class=16, with_gt=False)
+            result["served_1920x960"] = workloads.run_served(st, p, [], args.served_requests, args.seed + 8,
+                                                             views=(6, 6))
+            print(f"[bench] served 1920x960: {result['served_1920x960']}", file=sys.stderr, flush=True)
+            del st
     print(json.dumps(result))
     return 0
 

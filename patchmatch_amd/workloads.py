@@ -46,14 +46,15 @@ class Stream:
     def mask_id(self, c):
         return 1_000_000 * (c + 1) + 999_999
 
-    def draw(self, rng, n):
-        """n request specs: (class, ref index, neighbour indices)."""
+    def draw(self, rng, n, views=(2, 10)):
+        """n request specs: (class, ref index, neighbour indices), with
+        U{views[0]..views[1]} nearest neighbours."""
         w = np.array([c[2] for c in self.classes], dtype=np.float64)
         cls = rng.choice(len(self.classes), size=n, p=w / w.sum())
         out = []
         for c in cls:
             k = int(rng.integers(0, self.per_class))
-            nv = int(rng.integers(2, 11))
+            nv = int(rng.integers(views[0], views[1] + 1))
             others = sorted((m for m in range(self.per_class) if m != k), key=lambda m: (abs(m - k), m))
             out.append((int(c), k, others[:nv]))
         return out
@@ -65,6 +66,27 @@ class Stream:
                     view_ids=[self.image_id(c, m) for m in nb], view_poses=[s["poses"][m] for m in nb],
                     views=[s["images"][m] for m in nb], mask_id=self.mask_id(c), mask=s["mask"],
                     gt=s["gt"][k], shape=s["images"][k].shape)
+
+
+def digest(maps) -> str:
+    """Hash of a keyframe's final maps (plane, depth, score, nghbr) bit for bit."""
+    import hashlib
+    h = hashlib.blake2b(digest_size=16)
+    for a in maps[:4]:
+        h.update(np.ascontiguousarray(a).tobytes())
+    return h.hexdigest()
+
+
+def plain_digests(ctx, kfs, params):
+    """The maps of each keyframe densified alone (pmp_densify, images pooled), hashed."""
+    from ._lib import PMP_MAPS_FINAL
+    out = []
+    for kf in kfs:
+        ctx.set_views(kf["ref_id"], kf["ref_pose"], kf["view_ids"], kf["view_poses"])
+        ctx.set_mask(kf["mask_id"])
+        ctx.densify(params)
+        out.append(digest(ctx.get_maps(PMP_MAPS_FINAL)))
+    return out
 
 
 def accuracy(depth, gt, mask):
@@ -115,10 +137,15 @@ def run_batched(stream: Stream, params, n: int, seed: int):
     ctx.batch_densify(params, descs)
     ctx.synchronize()
     t_batch = time.perf_counter() - t
-    last_batch = ctx.batch_get_maps(n - 1, kfs[-1]["shape"])
-    same = all(np.array_equal(np.ascontiguousarray(a).view(np.uint32), np.ascontiguousarray(b).view(np.uint32))
-               for a, b in zip(last_plain, last_batch))
-    acc = [accuracy(ctx.batch_get_maps(k, kfs[k]["shape"])[1], kfs[k]["gt"], kfs[k]["mask"]) for k in range(n)]
+    # every keyframe of the batch against the same keyframe densified alone (after the timing)
+    acc, batch_dig = [], []
+    for k in range(n):
+        maps = ctx.batch_get_maps(k, kfs[k]["shape"])
+        batch_dig.append(digest(maps))
+        acc.append(accuracy(maps[1], kfs[k]["gt"], kfs[k]["mask"]))
+    plain_dig = plain_digests(ctx, kfs, params)
+    identical = sum(a == b for a, b in zip(plain_dig, batch_dig))
+    same = batch_dig[-1] == digest(last_plain)
     ctx.close()
     res = {}
     for W, H, _ in stream.classes:
@@ -129,6 +156,7 @@ def run_batched(stream: Stream, params, n: int, seed: int):
         "plain_depth_maps_per_s": round(n / t_plain, 3), "batched_depth_maps_per_s": round(n / t_batch, 3),
         "speedup": round(t_plain / t_batch, 3), "plain_s": round(t_plain, 3), "batched_s": round(t_batch, 3),
         "autotuned_config": cfg, "last_keyframe_bit_identical": bool(same),
+        "keyframes_bit_identical_to_plain": int(identical),
         "accuracy_mean": {k: round(float(np.mean([a[k] for a in acc if a[k] is not None])), 4)
                           for k in ("valid_fraction", "absrel_le_1pct_of_valid", "median_absrel")},
     }
@@ -345,10 +373,21 @@ def run_served_stream(stream: Stream, params, specs, arrivals, max_slots=16, hbm
     return out
 
 
-def run_served(stream: Stream, params, rates, n_requests: int, seed: int):
-    """The same request list at every rate; only the arrival times change."""
+def run_served(stream: Stream, params, rates, n_requests: int, seed: int, views=(2, 10), below=0.5):
+    """The same request list at every rate; only the arrival times change.
+    First every request arrives at once: the completion rate of that
+    saturated run is each form's capacity.  Then the given rates, and
+    `below` x the served capacity, where arrivals do not outrun the device
+    and the latencies are those of the requests rather than of a backlog."""
     out = {}
-    specs = stream.draw(np.random.default_rng(seed), n_requests)
+    specs = stream.draw(np.random.default_rng(seed), n_requests, views)
+    at_once = np.zeros(n_requests)
+    cap_plain = run_plain_stream(stream, params, specs, at_once)
+    cap_served = run_served_stream(stream, params, specs, at_once)
+    out["capacity"] = {"plain_per_s": cap_plain["throughput_per_s"], "served_per_s": cap_served["throughput_per_s"],
+                       "ratio": round(cap_served["throughput_per_s"] / cap_plain["throughput_per_s"], 3)}
+    print(f"[workloads] capacity: {out['capacity']}", file=sys.stderr, flush=True)
+    rates = list(rates) + ([round(below * cap_served["throughput_per_s"], 2)] if below else [])
     for rate in rates:
         arr = _arrivals(np.random.default_rng(seed + int(rate * 1000)), rate, n_requests)
         plain = run_plain_stream(stream, params, specs, arr)
@@ -359,3 +398,57 @@ def run_served(stream: Stream, params, rates, n_requests: int, seed: int):
                                   "throughput_ratio": round(served["throughput_per_s"] / plain["throughput_per_s"], 3),
                                   "p99_ratio": round(served["p99_ms"] / plain["p99_ms"], 3)}
     return out
+
+
+def image_bytes(stream: Stream, specs):
+    """Pool bytes the requests' distinct images need (1 B per pixel, 3 B once
+    packed as a neighbour), and the largest need of one request."""
+    need, packed, one = {}, set(), 0
+    for c, k, nb in specs:
+        W, H = stream.classes[c][0], stream.classes[c][1]
+        ids = [(stream.image_id(c, k), False), (stream.mask_id(c), False)] + \
+            [(stream.image_id(c, m), True) for m in nb]
+        one = max(one, sum(W * H * (3 if pk else 1) for _, pk in ids))
+        for iid, pk in ids:
+            need[iid] = W * H
+            if pk:
+                packed.add(iid)
+    return sum(b * (3 if iid in packed else 1) for iid, b in need.items()), one
+
+
+def verify_served(stream: Stream, params, specs, max_slots=16, hbm_budget=64 << 30, window=None):
+    """Every request of `specs`, submitted in order to a pmp_server with the
+    given slots and HBM budget and at most `window` requests outstanding
+    (default max_slots + 1), against the same keyframe densified alone
+    (pmp_densify): the number of bit-identical results, and the server's
+    evictions and uploads."""
+    from ._lib import Context, Server
+    ctx = Context(0)
+    kfs = [stream.keyframe(sp) for sp in specs]
+    want, put = [], set()
+    for kf in kfs:
+        for iid, img in [(kf["ref_id"], kf["ref"])] + list(zip(kf["view_ids"], kf["views"])) + \
+                [(kf["mask_id"], kf["mask"])]:
+            if iid not in put:
+                ctx.image_put(iid, img)
+                put.add(iid)
+        want.extend(plain_digests(ctx, [kf], params))
+    ctx.close()
+    srv = Server(0, max_slots=max_slots, hbm_budget=hbm_budget, edge_mode=params.edge_mode)
+    window = window or max_slots + 1
+    tickets, got = [], []
+    for q, kf in enumerate(kfs):
+        if len(tickets) - len(got) >= window:
+            srv.wait(tickets[len(got)])
+            got.append(digest(srv.fetch(tickets[len(got)])))
+        tickets.append(srv.submit(params, kf["ref_id"], kf["ref_pose"], kf["view_ids"], kf["view_poses"],
+                                  kf["shape"], ref=kf["ref"], views=kf["views"], mask_id=kf["mask_id"],
+                                  mask=kf["mask"]))
+    while len(got) < len(tickets):
+        srv.wait(tickets[len(got)])
+        got.append(digest(srv.fetch(tickets[len(got)])))
+    st = srv.stats()
+    srv.close()
+    return {"requests": len(specs), "bit_identical_to_plain": int(sum(a == b for a, b in zip(want, got))),
+            "evictions": st["evictions"], "uploads": st["uploads"], "max_in_flight": st["max_in_flight"],
+            "hbm_budget": hbm_budget}

@@ -13,7 +13,7 @@ from patchmatch_amd._lib import PMP_MAPS_FINAL
 from patchmatch_amd.params import (AGG_FOLD_INDEP, AGG_MAX, AGG_REF_FOLD, AGG_TOPK_MEAN, COVER_FULL, EDGE_CLAMP,
                                    EDGE_ZERO, config0_params, config1_params, default_params)
 
-from conftest import random_planes
+from conftest import cached, random_planes
 
 pytestmark = pytest.mark.gpu
 
@@ -181,31 +181,34 @@ def test_propagate_view_counts(gpu_ctx, n_views, agg):
     assert np.count_nonzero(od != d_init) > 500
 
 
-def test_propagate_full_size_row_band(gpu_ctx):
-    """The 1920x960 keyframe with 6 neighbours at full size: the GPU's first
-    red-black iteration from its own initial maps, against the oracle
-    recomputing a band of rows from the same initial maps (red pass on the band
-    plus the 3-row reach of the candidates, then the black pass on the band).
-    Rows of one colour only read the other colour within a pass, so the band
-    equals the whole pass there: bit for bit."""
+@pytest.mark.parametrize("it", [0, 5])
+def test_propagate_full_size_row_band(gpu_ctx, it):
+    """The 1920x960 keyframe with 6 neighbours at full size: the GPU's
+    iteration `it` (the first, and the sixth of eight) from its own maps before
+    it, against the oracle recomputing a band of rows from the same maps (red
+    pass on the band plus the 3-row reach of the candidates, then the black
+    pass on the band).  Rows of one colour only read the other colour within
+    a pass, so the band equals the whole pass there: bit for bit."""
     from patchmatch_amd import synth
-    kf = synth.config1(0, W=1920, H=960, n_views=6)
+    kf = cached("c1_full", lambda: synth.config1(0, W=1920, H=960, n_views=6))
     p = config1_params(seed=0)
     _load(gpu_ctx, kf)
     gpu_ctx.random_initialization(p)
+    if it:
+        gpu_ctx.red_black_pass(p, 0, it)
     ip, idp, isc = gpu_ctx.get_maps(0)
-    gpu_ctx.red_black_pass(p, 0, 1)
+    gpu_ctx.red_black_pass(p, it, it + 1)
     gp, gd, gs = gpu_ctx.get_maps(0)
     ok = or_py.Keyframe.from_synth(kf)
     op, od, os_ = (np.array(a, copy=True) for a in (ip, idp, isc))
     r0, r1 = 470, 506
-    or_py.propagate_rows(ok, p, op, od, os_, 0, 0, r0 - 3, r1 + 3)
-    or_py.propagate_rows(ok, p, op, od, os_, 0, 1, r0, r1)
+    or_py.propagate_rows(ok, p, op, od, os_, it, 0, r0 - 3, r1 + 3)
+    or_py.propagate_rows(ok, p, op, od, os_, it, 1, r0, r1)
     band = slice(r0, r1)
     for name, g, o in (("plane", gp[band], op[band]), ("depth", gd[band], od[band]), ("score", gs[band], os_[band])):
-        eq = _report(f"1920x960 rows {r0}-{r1} {name}", g, o) | (np.isnan(g) & np.isnan(o))
+        eq = _report(f"1920x960 iteration {it} rows {r0}-{r1} {name}", g, o) | (np.isnan(g) & np.isnan(o))
         assert eq.all(), f"{name}: {np.count_nonzero(~eq)} of {eq.size} differ"
-    assert np.count_nonzero(od[band] != idp[band]) > 0.3 * od[band].size
+    assert np.count_nonzero(od[band] != idp[band]) > (0.3 if it == 0 else 0.02) * od[band].size
 
 
 def test_propagate_largest_served_size_row_band(gpu_ctx):
