@@ -407,6 +407,17 @@ int pmp_clean(pmp_ctx* ctx, int32_t width, int32_t height, int32_t n_maps, const
  *     most max_normal_deg apart; plane4_maps (n_maps host pointers, W*H*4
  *     floats as pmp_get_maps returns them) are then required.
  * Values <= 0 switch a test off; with both off this is pmp_clean. */
+/* pmp_clean without allocation.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under graph capture,
+ * tight serving loops).  Same arguments and results as pmp_clean; the staged
+ * maps, the cleaned map and the bearing tables live in one file-scope device
+ * array of 16 MiB per GPU (held while the library is loaded), so the call
+ * makes no device allocation.  PMP_EINVAL when (n_maps + 1) * width * height
+ * + 2 * (width + height) exceeds 4194304 floats (640x320 takes any n_maps up
+ * to 16, 1280x640 up to 4).  Calls on one GPU share the array: calls that
+ * overlap on one GPU race. */
+int pmp_clean_noalloc(pmp_ctx* ctx, int32_t width, int32_t height, int32_t n_maps, const float* const* depth_maps,
+                      const pmp_pose* poses, float same_depth_thr, int32_t min_consistent, float* cleaned);
 int pmp_clean_ex(pmp_ctx* ctx, int32_t width, int32_t height, int32_t n_maps, const float* const* depth_maps,
                  const float* const* plane4_maps, const pmp_pose* poses, float same_depth_thr, int32_t min_consistent,
                  float max_reproj_px, float max_normal_deg, float* cleaned);

@@ -51,7 +51,7 @@ EXPORTS = [
     "pmp_server_cancel", "pmp_server_reset",
     "pmp_preinit_landmarks", "pmp_memorize_result", "pmp_preinit_warp", "pmp_forget_result",
     "pmp_write_dense_msgpack", "pmp_write_ply", "pmp_image_put_thumbnail", "pmp_resize", "pmp_image_get",
-    "pmp_score_view", "pmp_depth_interval", "pmp_image_size",
+    "pmp_score_view", "pmp_depth_interval", "pmp_image_size", "pmp_clean_noalloc",
 ]
 
 PMP_MAX_VIEWS = 10
@@ -165,6 +165,7 @@ def load():
         "pmp_eval_counts": ([_vp, _i64p, _i64p], c_int),
         "pmp_clean": ([_vp, i32, i32, i32, P(_f32p), P(Pose), f32, i32, _f32p], c_int),
         "pmp_clean_ex": ([_vp, i32, i32, i32, P(_f32p), P(_f32p), P(Pose), f32, i32, f32, f32, _f32p], c_int),
+        "pmp_clean_noalloc": ([_vp, i32, i32, i32, P(_f32p), P(Pose), f32, i32, _f32p], c_int),
         "pmp_pruner_add_view": ([_vp, _f32p, P(Pose), _u8p, i32, i32], c_int),
         "pmp_pruner_prune": ([_vp, f32, _f32p, i64, _f64p, _u8p, _i32p, _i64p], c_int),
         "pmp_pruner_pop_head": ([_vp], c_int),
@@ -495,6 +496,17 @@ class Context:
         out = np.zeros((H, W), np.float32)
         self._chk(self.L.pmp_clean(self.h, W, H, len(depths), arr, ps, thr, min_consistent, _p(out, _f32p)),
                   "pmp_clean")
+        return out
+
+    def clean_noalloc(self, depths, poses, thr, min_consistent):
+        """pmp_clean_noalloc: clean() with no device allocation (a 16 MiB device array per GPU)."""
+        depths = [np.ascontiguousarray(d, dtype=np.float32) for d in depths]
+        H, W = depths[0].shape
+        arr = (_f32p * len(depths))(*[_p(d, _f32p) for d in depths])
+        ps = (Pose * len(depths))(*[Pose.from12(p) for p in poses])
+        out = np.zeros((H, W), np.float32)
+        self._chk(self.L.pmp_clean_noalloc(self.h, W, H, len(depths), arr, ps, thr, min_consistent, _p(out, _f32p)),
+                  "pmp_clean_noalloc")
         return out
 
     def clean_ex(self, depths, planes, poses, thr, min_consistent, max_reproj_px=0.0, max_normal_deg=0.0):

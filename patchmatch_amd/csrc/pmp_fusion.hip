@@ -23,6 +23,7 @@
 #include <string.h>
 
 #include <climits>
+#include <mutex>
 #include <vector>
 
 #include "pmp_internal.h"
@@ -259,6 +260,54 @@ __global__ __launch_bounds__(kScanBlock) void k_prune_emit(FuseArgs a, const flo
     kps[2 * (size_t)pos + 1] = j;
 }
 
+// ---- the Cleaner without allocation (pmp_clean_noalloc)
+// Its device memory is one file-scope array per GPU: the staged depth maps,
+// the cleaned map and the two bearing tables of the call.  In host code the
+// array's name is only a stand-in: the device address comes from
+// hipGetSymbolAddress, looked up once per device and kept; only that address
+// reaches kernels and copies.  Every call on one GPU uses the same array, so
+// calls that overlap on one GPU race.
+constexpr size_t kCleanScratchFloats = size_t(1) << 22;   // 16 MiB per GPU
+static __device__ __align__(256) float scratch[kCleanScratchFloats];
+
+constexpr int kScratchDevices = 64;
+static std::mutex g_scratch_mu;
+static float* g_scratch[kScratchDevices] = {};
+
+// The device address of this GPU's scratch array (ctx's device current).
+static int clean_scratch(pmp_ctx* ctx, float*& out) {
+    if (ctx->device < 0 || ctx->device >= kScratchDevices)
+        return set_error(ctx, PMP_EINVAL, "device ordinal beyond the scratch table");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch[ctx->device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) return check_hip(ctx, e != hipSuccess ? e : hipErrorInvalidSymbol,
+                                                    "hipGetSymbolAddress(scratch)");
+        g_scratch[ctx->device] = (float*)p;
+    }
+    out = g_scratch[ctx->device];
+    return PMP_OK;
+}
+
+// Writes the W + H bearing-table entries of a W x H image to the device
+// tables, through a fixed host buffer (no allocation).
+static int write_tables(pmp_ctx* ctx, int W, int H, float2* rowtab, float2* coltab) {
+    constexpr int kChunk = 512;
+    float2 buf[kChunk];
+    for (int side = 0; side < 2; ++side) {
+        const int n = side == 0 ? H : W;
+        float2* dst = side == 0 ? rowtab : coltab;
+        for (int k0 = 0; k0 < n; k0 += kChunk) {
+            const int m = n - k0 < kChunk ? n - k0 : kChunk;
+            for (int k = 0; k < m; ++k) buf[k] = side == 0 ? table_row(k0 + k, H) : table_col(k0 + k, W);
+            PMP_HIP(ctx, hipMemcpyAsync(dst + k0, buf, m * sizeof(float2), hipMemcpyHostToDevice, ctx->stream));
+            PMP_HIP(ctx, hipStreamSynchronize(ctx->stream));   // buf is rewritten next
+        }
+    }
+    return PMP_OK;
+}
+
 static int geo_args(pmp_ctx* ctx, int W, int H, GeoArgs& g) {
     Tables tb;
     int rc = get_tables(ctx, W, H, ctx->last_sigma, tb);
@@ -349,6 +398,60 @@ int pmp_clean_ex(pmp_ctx* ctx, int32_t width, int32_t height, int32_t n_maps, co
                  float max_reproj_px, float max_normal_deg, float* cleaned) {
     return clean_impl(ctx, width, height, n_maps, depth_maps, plane4_maps, poses, same_depth_thr, min_consistent,
                       max_reproj_px, max_normal_deg, cleaned);
+}
+
+int pmp_clean_noalloc(pmp_ctx* ctx, int32_t width, int32_t height, int32_t n_maps, const float* const* depth_maps,
+                      const pmp_pose* poses, float same_depth_thr, int32_t min_consistent, float* cleaned) {
+    if (!ctx || !depth_maps || !poses || !cleaned || width < 2 || height < 2) return PMP_EINVAL;
+    if (n_maps < 1 || n_maps > kMaxMaps) return set_error(ctx, PMP_EINVAL, "n_maps must be in [1, 16]");
+    const size_t P = (size_t)width * height;
+    const size_t tab_off = ((size_t)(n_maps + 1) * P + 1) / 2 * 2;   // float2 tables start 8-byte aligned
+    const size_t need = tab_off + 2 * ((size_t)width + (size_t)height);
+    if (need > kCleanScratchFloats)
+        return set_error(ctx, PMP_EINVAL,
+                         "pmp_clean_noalloc holds at most 4194304 floats ((n_maps + 1) * width * height + "
+                         "2 * (width + height)); use pmp_clean");
+    for (int o = 0; o < n_maps; ++o)
+        if (!depth_maps[o]) return set_error(ctx, PMP_EINVAL, "null depth map");
+    PMP_HIP(ctx, hipSetDevice(ctx->device));
+    float* base = nullptr;
+    int rc = clean_scratch(ctx, base);
+    if (rc) return rc;
+    float* dout = base + (size_t)n_maps * P;
+    float2* rowtab = (float2*)(base + tab_off);
+    float2* coltab = rowtab + height;
+    rc = write_tables(ctx, width, height, rowtab, coltab);
+    if (rc) return rc;
+    FuseArgs a;
+    memset(&a, 0, sizeof(a));
+    a.g.W = width;
+    a.g.H = height;
+    a.g.rowtab = rowtab;
+    a.g.coltab = coltab;
+    a.g.ku = (float)((double)width / (2.0 * M_PI));
+    a.g.hu = (float)(0.5 * width);
+    a.g.kv = (float)((double)height / M_PI);
+    a.g.hv = (float)(0.5 * height);
+    a.n = n_maps;
+    a.thr = same_depth_thr;
+    for (int o = 0; o < n_maps; ++o) {
+        memcpy(a.poses[o], poses[o].R, 9 * sizeof(double));
+        memcpy(a.poses[o] + 9, poses[o].t, 3 * sizeof(double));
+        PMP_HIP(ctx, hipMemcpyAsync(base + (size_t)o * P, depth_maps[o], P * sizeof(float), hipMemcpyHostToDevice,
+                                    ctx->stream));
+    }
+    CleanChecks c;
+    c.planes = nullptr;
+    c.px2 = 0.0f;
+    c.iw = (float)(1.0 / width);
+    c.ih = (float)(1.0 / height);
+    c.cos2 = 0.0;
+    dim3 blk(256), grd((width + 255) / 256, height);
+    hipLaunchKernelGGL(k_clean, grd, blk, 0, ctx->stream, a, c, (const float*)base, min_consistent, dout);
+    PMP_HIP(ctx, hipGetLastError());
+    PMP_HIP(ctx, hipMemcpyAsync(cleaned, dout, P * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    PMP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return PMP_OK;
 }
 
 int pmp_pruner_add_view(pmp_ctx* ctx, const float* cleaned_depth, const pmp_pose* pose, const uint8_t* bgr,
