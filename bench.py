@@ -49,7 +49,7 @@ HBM_PEAK_GBS = 8000.0
 # L2-to-memory bytes per k_propagate_q launch on the 1920x960 + 6-view
 # workload, from rocprofv3 FETCH_SIZE and WRITE_SIZE scaled by a copy of known
 # size (profiles/k_propagate_q_c1_1920x960_6views.md); None for other workloads.
-TRAFFIC_BYTES_PER_LAUNCH_C1 = int((370_600 * 2.00 + 42_982 * 1.00) * 1024)
+TRAFFIC_BYTES_PER_LAUNCH_C1 = int((361_482 * 2.00 + 42_982 * 1.00) * 1024)
 TRAFFIC_SOURCE = ("profile-derived, not measured in this run: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE of "
                   "k_propagate_q on this workload, scaled by a streaming copy of known size "
                   "(profiles/k_propagate_q_c1_1920x960_6views.md)")
@@ -325,12 +325,13 @@ def main(argv=None):
             rates = [float(r) for r in args.rates.split(",") if r]
             result["served"] = workloads.run_served(st, p, rates, args.served_requests, args.seed + 4)
             print(f"[bench] served: {result['served']}", file=sys.stderr, flush=True)
-            # every request of the stream against plain, under a budget that evicts images
+            # every request of the stream against plain, under a budget that holds the maps of two
+            # slots of its largest request and the images of one request: pooled images are evicted
             specs = st.draw(np.random.default_rng(args.seed + 6), 16)
-            big = 5760 * 2880
+            big = max(st.classes[c][0] * st.classes[c][1] for c, _, _ in specs)
             _, one = workloads.image_bytes(st, specs)
             result["served"]["verification"] = workloads.verify_served(
-                st, p, specs, max_slots=2, hbm_budget=2 * 84 * big + 2 * one + (64 << 20), window=2)
+                st, p, specs, max_slots=2, hbm_budget=2 * 84 * big + one + (64 << 20), window=2)
             print(f"[bench] served verification: {result['served']['verification']}", file=sys.stderr, flush=True)
             del st
             # the plain workload's shape alone (1920x960, 6 neighbours) below capacity: the latency target's case

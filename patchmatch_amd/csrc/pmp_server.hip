@@ -122,6 +122,7 @@ class GpuExec final : public sched::Executor {
     }
 
     void drop(uint64_t id) override {
+        (void)hipSetDevice(ctx->device);   // also called from pmp_server_reset's thread
         std::lock_guard<std::mutex> lk(pool_mu);
         auto it = ctx->images.find(id);
         if (it == ctx->images.end()) return;
@@ -154,8 +155,9 @@ class GpuExec final : public sched::Executor {
         bool post = false;
         std::vector<const sched::StepItem*> fin, seeds;
         int rc = PMP_OK;
+        // the descriptors read the pool, and errors are reported through the context
+        std::lock_guard<std::mutex> lk(pool_mu);
         {
-            std::lock_guard<std::mutex> lk(pool_mu);   // the descriptors read the pool
             if (desc_begin(ctx, d) != PMP_OK) {
                 err = ctx->last_error;
                 return PMP_EDEVICE;
