@@ -325,13 +325,20 @@ def main(argv=None):
             rates = [float(r) for r in args.rates.split(",") if r]
             result["served"] = workloads.run_served(st, p, rates, args.served_requests, args.seed + 4)
             print(f"[bench] served: {result['served']}", file=sys.stderr, flush=True)
-            # every request of the stream against plain, under a budget that holds the maps of two
-            # slots of its largest request and the images of one request: pooled images are evicted
+            # every request of a stream against plain, under a budget that holds the maps of two
+            # 5760x2880 slots and the images of two requests. Requests 0 and 1 are 5760x2880 so that
+            # both slots grow at once; with 8 and 13 they cover four disjoint 4-neighbour image sets,
+            # more than that room, so pooled images must be evicted.
             specs = st.draw(np.random.default_rng(args.seed + 6), 16)
-            big = max(st.classes[c][0] * st.classes[c][1] for c, _, _ in specs)
-            _, one = workloads.image_bytes(st, specs)
+            c_big = len(st.classes) - 1
+            for q, k in ((0, 10), (1, 40), (8, 70), (13, 100)):
+                specs[q] = (c_big, k, [k - 2, k - 1, k + 1, k + 2])
+            big = st.classes[c_big][0] * st.classes[c_big][1]
+            total, one = workloads.image_bytes(st, specs)
+            room = 2 * one + (64 << 20)
             result["served"]["verification"] = workloads.verify_served(
-                st, p, specs, max_slots=2, hbm_budget=2 * 84 * big + one + (64 << 20), window=2)
+                st, p, specs, max_slots=2, hbm_budget=2 * 84 * big + room, window=2)
+            result["served"]["verification"].update(image_bytes=total, image_room=room)
             print(f"[bench] served verification: {result['served']['verification']}", file=sys.stderr, flush=True)
             del st
             # the plain workload's shape alone (1920x960, 6 neighbours) below capacity: the latency target's case

@@ -145,15 +145,23 @@ void Server::retire(Req* r) {
 // in-flight pixel budget.  Caller holds mu_.
 void Server::admit() {
     while (!queue_.empty() && !paused_) {
+        Req* r = queue_.front();
+        const int64_t P = (int64_t)r->job.W * r->job.H;
+        // best fit among the free slots: the smallest whose maps already hold
+        // P pixels, else the largest (least growth), so that one large request
+        // does not grow every slot's maps in turn
         int slot = -1;
         int64_t inflight = 0;
         for (int k = 0; k < o_.max_slots; ++k) {
-            if (slot_req_[k]) inflight += (int64_t)slot_req_[k]->job.W * slot_req_[k]->job.H;
-            else if (slot < 0) slot = k;
+            if (slot_req_[k]) {
+                inflight += (int64_t)slot_req_[k]->job.W * slot_req_[k]->job.H;
+                continue;
+            }
+            if (slot < 0) { slot = k; continue; }
+            const int64_t c = slot_cap_[k], b = slot_cap_[slot];
+            if (c >= P ? (b < P || c < b) : (b < P && c > b)) slot = k;
         }
         if (slot < 0) return;
-        Req* r = queue_.front();
-        const int64_t P = (int64_t)r->job.W * r->job.H;
         if (o_.inflight_kpx > 0 && inflight > 0 && inflight + P > 1000ll * o_.inflight_kpx) return;
         if (P > slot_cap_[slot]) {
             const int64_t grow = ex_->map_bytes(P) - ex_->map_bytes(slot_cap_[slot]);

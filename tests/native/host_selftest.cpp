@@ -561,6 +561,43 @@ static void test_destroy_finishes() {
     CHECK(ex.violations == 0);
 }
 
+// 6. Slot choice: a request takes the smallest free slot whose maps already
+// hold it, else the largest free slot, so slot maps grow only when no free
+// slot is large enough.
+static void test_slot_best_fit() {
+    const int Ws = 16, Hs = 8, Wb = 64, Hb = 32;
+    SimExec ex(1ll << 30, 2);
+    sched::Options o;
+    o.max_slots = 2;
+    o.budget = 1ll << 30;
+    Images im;
+    sched::Server s(&ex, o);
+    std::string err;
+    uint64_t a, b, c, d;
+    s.pause(true);   // admit the small and the large request together
+    CHECK(s.submit(request(im, 1, 10, 2, Ws, Hs), params(1, 1), &a, err) == PMP_OK);
+    CHECK(s.submit(request(im, 2, 20, 2, Wb, Hb), params(1, 2), &b, err) == PMP_OK);
+    s.pause(false);
+    CHECK(fetch(s, a, Ws, Hs) == PMP_OK);
+    CHECK(fetch(s, b, Wb, Hb) == PMP_OK);
+    {
+        std::lock_guard<std::mutex> lk(ex.mu);
+        CHECK(ex.slot_grows == 2);
+        CHECK(ex.slot_pixels[0] == (int64_t)Ws * Hs && ex.slot_pixels[1] == (int64_t)Wb * Hb);
+    }
+    // both slots free: the large request takes slot 1, the small one slot 0
+    CHECK(s.submit(request(im, 3, 30, 2, Wb, Hb), params(1, 3), &c, err) == PMP_OK);
+    CHECK(fetch(s, c, Wb, Hb) == PMP_OK);
+    CHECK(s.submit(request(im, 4, 40, 2, Ws, Hs), params(1, 4), &d, err) == PMP_OK);
+    CHECK(fetch(s, d, Ws, Hs) == PMP_OK);
+    {
+        std::lock_guard<std::mutex> lk(ex.mu);
+        CHECK(ex.slot_grows == 2);
+    }
+    for (uint64_t t : {a, b, c, d}) stages_complete(ex, t, 1);
+    CHECK(ex.violations == 0);
+}
+
 // ---------------------------------------------------------------- host-only library code
 static void test_frontend_and_writers() {
     // depth interval: 1e-5 and 1.1 x the distance at sorted index n/10*9
@@ -635,6 +672,7 @@ int main(int argc, char** argv) {
     test_reset();
     test_faults();
     test_destroy_finishes();
+    test_slot_best_fit();
     if (!sched_only) test_frontend_and_writers();
     printf("host_selftest: ok\n");
     return 0;

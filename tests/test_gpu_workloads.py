@@ -58,15 +58,18 @@ def test_served_stream_with_evictions_equals_plain():
     p = config1_params(seed=4)
     rng = np.random.default_rng(14)
     specs = st.draw(rng, 16)
-    # four 5760x2880 requests over different images, 2 to 4 views
-    specs[3], specs[7], specs[11], specs[14] = (3, 2, [1, 3, 4]), (3, 9, [8, 10, 7]), (3, 13, [12, 14, 15, 11]), \
+    # four 5760x2880 requests over different images, 2 to 4 views; the first
+    # two open the stream, so both slots grow to 5760x2880 at once
+    specs[0], specs[1], specs[11], specs[14] = (3, 2, [1, 3, 4]), (3, 9, [8, 10, 7]), (3, 13, [12, 14, 15, 11]), \
         (3, 5, [4, 6])
     big = 5760 * 2880
     total, one = workloads.image_bytes(st, specs)
     # maps of two 5760x2880 slots plus the images of the two requests that
-    # may be outstanding at once: well below the stream's images, so pooled
+    # may be outstanding at once: once both slots have grown, the pool holds
+    # at most `room`, below the four large requests' images alone, so pooled
     # images must be evicted
     room = 2 * one + (64 << 20)
+    assert workloads.image_bytes(st, [specs[q] for q in (0, 1, 11, 14)])[0] > room
     assert total > 1.3 * room
     res = workloads.verify_served(st, p, specs, max_slots=2, hbm_budget=2 * 84 * big + room, window=2)
     print(f"[served stream] images {total / 1e9:.2f} GB, room {room / 1e9:.2f} GB: {res}", flush=True)
