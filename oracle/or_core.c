@@ -1200,7 +1200,7 @@ EXPORT int or_score_at_coords_f64(const or_keyframe* kf, const or_params* p, con
 /* diagnostics for tests                                               */
 /* ------------------------------------------------------------------ */
 /* which: 0 rcp 1 rsqrt 2 sqrt 3 asin 4 acos 5 atan2(a,b) 6 exp 7 log 8 sin2pi 9 cos2pi
- * 10 latitude of the direction (b, a, b) */
+ * 10 latitude of the direction (b, a, b) 11 longitude of the direction (a, 0, b) */
 EXPORT int or_eval_math(int which, int64_t n, const float* a, const float* b, float* out) {
     for (int64_t k = 0; k < n; ++k) {
         float x = a[k], r = 0.0f, s, c;
@@ -1216,6 +1216,7 @@ EXPORT int or_eval_math(int which, int64_t n, const float* a, const float* b, fl
             case 8: or_sincos2pi(x, &s, &c); r = s; break;
             case 9: or_sincos2pi(x, &s, &c); r = c; break;
             case 10: or_lonlat(b[k], x, b[k], &s, &r); break; /* latitude of (b, a, b) */
+            case 11: or_lonlat(x, 0.0f, b[k], &r, &s); break;  /* longitude of (a, 0, b) */
             default: return -1;
         }
         out[k] = r;

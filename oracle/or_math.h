@@ -26,10 +26,10 @@
  *           pi - 2 asin(sqrt((1+x)/2)) for x<-1/2.
  *   atan2 : q = min/max in [0,1], atan(q) = q P(q^2), octant fix-up,
  *           sign of y.
- *   lonlat: longitude atan2(x, z), latitude atan2(y, sqrt(x^2+z^2)) through
- *           the same polynomial: rs = rsqrt(h^2 y^2) gives |y|/h = y^2 rs
- *           and h/|y| = h^2 rs; atan of the smaller, pi/2 minus it above 45
- *           degrees.
+ *   lonlat: r = rsqrt(h^2), h = sqrt(x^2+z^2); longitude atan2(x, z) from
+ *           the octant angle asin(min(|x|,|z|) r) = s + s^3 P(s^2) on
+ *           [0, 1/sqrt 2]; latitude atan2(y, h) = atan(|y| r), pi/2 minus
+ *           atan(1/(|y| r)) above 45 degrees, through atan's polynomial.
  *   exp   : Cody-Waite reduction x = k ln2 + r, |r| <= ln2/2, degree-6
  *           polynomial, scale by 2^k through the exponent field.
  *   log   : x = m 2^e, m in [sqrt(1/2), sqrt 2), log1p(m-1) = f P(f).
@@ -186,21 +186,43 @@ static inline float or_atan2(float y, float x) {
     return copysignf(a, y);
 }
 
+/* asin(s) for s in [0, 1/sqrt 2] (the octant angle of the longitude):
+ * s + s^3 P(s^2) */
+static inline float or_asin_oct(float s) {
+    float z = s * s;
+    float p = 1.016279534e-01f;
+    p = fmaf(p, z, -5.578331649e-02f);
+    p = fmaf(p, z, 6.111577526e-02f);
+    p = fmaf(p, z, 3.886268660e-02f);
+    p = fmaf(p, z, 7.548701018e-02f);
+    p = fmaf(p, z, 1.666531712e-01f);
+    return fmaf(s * z, p, s);
+}
+
 /* longitude atan2(x, z) and latitude atan2(y, h), h = sqrt(x^2 + z^2), of a
- * direction with finite components (|x|, |y|, |z| < 1e7): rs = rsqrt(h^2 y^2)
- * gives |y|/h = y^2 rs and h/|y| = h^2 rs; the latitude is atan of the
- * smaller (pi/2 minus it above 45 degrees, i.e. when y^2 > h^2).  h^2 y^2
- * is clamped to [1e-30, 1e30]; the ratio is not clamped (it exceeds 1 only
- * when y^2 = h^2 exactly, by 2^-23 at most). */
+ * direction with finite components (|x|, |y|, |z| < 1e7), from one
+ * r = rsqrt(h^2) (h^2 clamped to [1e-30, 1e30]):
+ *   longitude: the angle from the nearer of the z and x axes is
+ *     asin(min(|x|, |z|) r), in [0, pi/4]; octant fix-up as atan2;
+ *   latitude: atan(t), t = |y| r, or pi/2 - atan(1/t) above 45 degrees
+ *     (t > 1).  The ratio in [0, 1] is min(t, rcp(t)): the reciprocal is
+ *     below 1 for t > 1 and at least 1 for t < 1, and for t = 0 (or_rcp_pos
+ *     gives NaN) or_min returns t.
+ * x = z = 0 gives the longitude +-0 and the latitude +-pi/2 (t = |y| 1e15);
+ * y = 0 the latitude +-0.  Below h^2 = 1e-30 (only within 1e-15 of a pole)
+ * the clamped r makes the longitude approximate. */
 static inline void or_lonlat(float x, float y, float z, float* lon, float* lat) {
+    float ax = fabsf(x), az = fabsf(z);
     float h2 = fmaf(x, x, z * z);
-    float y2 = y * y;
-    float rs = or_rsqrt_pos(or_med3(h2 * y2, 1e-30f, 1e30f));
-    int big = y2 > h2;
-    float a = or_atan_core((big ? h2 : y2) * rs);
-    if (big) a = OR_PIO2_F - a;
-    *lat = copysignf(a, y);
-    *lon = or_atan2(x, z);
+    float r = or_rsqrt_pos(or_med3(h2, 1e-30f, 1e30f));
+    float a = or_asin_oct(or_med3(ax, az, 0.0f) * r);
+    if (ax > az) a = OR_PIO2_F - a;
+    if (z < 0.0f) a = OR_PI_F - a;
+    *lon = copysignf(a, x);
+    float t = fabsf(y) * r;
+    float b = or_atan_core(or_min(t, or_rcp_pos(t)));
+    if (t > 1.0f) b = OR_PIO2_F - b;
+    *lat = copysignf(b, y);
 }
 
 /* exp(x); x <= -87.3 -> 0, x >= 88 -> e^88 clamp; NaN -> e^-87.3 path (0) */

@@ -279,7 +279,7 @@ def prune(depths, poses, bgr, thr):
 
 
 MATH = {"rcp": 0, "rsqrt": 1, "sqrt": 2, "asin": 3, "acos": 4, "atan2": 5, "exp": 6, "log": 7,
-        "sin2pi": 8, "cos2pi": 9, "lat": 10}
+        "sin2pi": 8, "cos2pi": 9, "lat": 10, "lon": 11}
 
 
 def eval_math(name, a, b=None):

@@ -5,6 +5,8 @@ refined towards minimax by Lawson's iteration (iteratively reweighted least
 squares), in double precision, then rounded to float:
 
   asin(a) = a + a^3 P(a^2),  a in [0, 1/2]        P degree 4 (rel. weight a^2)
+  asin(s) = s + s^3 P(s^2),  s in [0, 1/sqrt 2]   P degree 5 (rel. weight s^2;
+                                                   the longitude's octant angle)
   atan(q) = q P(q^2),        q in [0, 1]          P degree 8
   exp(r)  = P(r),            r in [-ln2/2, ln2/2] P degree 6
   log1p(f) = f P(f),         f in [sqrt(1/2)-1, sqrt(2)-1]  P degree 9
@@ -51,6 +53,8 @@ def coefficients():
     c = {}
     c["asin"] = _fit(_safe(lambda z: (np.arcsin(np.sqrt(z)) - np.sqrt(z)) / (z * np.sqrt(z)),
                            lambda z: 1 / 6 + 3 / 40 * z), 0, 0.25, 4, weight=lambda z: z)
+    c["asin_oct"] = _fit(_safe(lambda z: (np.arcsin(np.sqrt(z)) - np.sqrt(z)) / (z * np.sqrt(z)),
+                               lambda z: 1 / 6 + 3 / 40 * z), 0, 0.5, 5, weight=lambda z: z)
     c["atan"] = _fit(_safe(lambda z: np.arctan(np.sqrt(z)) / np.sqrt(z), lambda z: 1 + 0 * z), 0, 1, 8)
     c["exp"] = _fit(np.exp, -np.log(2) / 2, np.log(2) / 2, 6)
     lo, hi = np.sqrt(0.5) - 1, np.sqrt(2) - 1
@@ -62,4 +66,4 @@ def coefficients():
 
 if __name__ == "__main__":
     for name, v in coefficients().items():
-        print(f"{name:5s}", ", ".join("%.9ef" % x for x in v))
+        print(f"{name:8s}", ", ".join("%.9ef" % x for x in v))

@@ -16,8 +16,10 @@
 //          (Cody & Waite, Software Manual for the Elementary Functions, 1980)
 //   acos   from the same core polynomial, three ranges
 //   atan2  q = min/max, atan(q) = q P(q^2) on [0, 1], octant fix-up
-//   lonlat longitude atan2(x, z), latitude atan2(y, sqrt(x^2 + z^2)) through
-//          the same atan polynomial
+//   lonlat r = rsqrt(h^2), h = sqrt(x^2 + z^2); longitude atan2(x, z) from
+//          the octant angle asin(min(|x|, |z|) r) = s + s^3 P(s^2) on
+//          [0, 1/sqrt 2]; latitude atan2(y, h) = atan(|y| r), pi/2 minus
+//          atan(1/(|y| r)) above 45 degrees
 //   exp    Cody-Waite reduction by ln2, degree-6 polynomial, 2^k by exponent bits
 //   log    mantissa in [sqrt(1/2), sqrt 2), log1p(f) = f P(f)
 //   sincos2pi  quadrant from floor(4u), sin/cos(pi f / 2) in f^2
@@ -157,25 +159,44 @@ PMP_DF float atan2_(float y, float x) {
     return __builtin_copysignf(a, y);
 }
 
+// asin(s) for s in [0, 1/sqrt 2] (the octant angle of the longitude):
+// s + s^3 P(s^2)
+PMP_DF float asin_oct(float s) {
+    const float z = s * s;
+    float p = 1.016279534e-01f;
+    p = fma_(p, z, -5.578331649e-02f);
+    p = fma_(p, z, 6.111577526e-02f);
+    p = fma_(p, z, 3.886268660e-02f);
+    p = fma_(p, z, 7.548701018e-02f);
+    p = fma_(p, z, 1.666531712e-01f);
+    return fma_(s * z, p, s);
+}
+
 // Longitude atan2(x, z) and latitude atan2(y, h), h = sqrt(x^2 + z^2), of a
-// direction (x, y, z) with finite components (|x|, |y|, |z| < 1e7).  One
-// rsqrt of h^2 y^2 gives both ratios, |y| / h = y^2 rs and h / |y| = h^2 rs;
-// the latitude is atan of the smaller one (pi/2 minus it above 45 degrees,
-// i.e. when y^2 > h^2, which picks the ratio before it is formed).
-// h^2 y^2 is clamped to [1e-30, 1e30].  The ratio is not clamped: for
-// components below 1e7 the product stays below 1e30, a clamp raising a tiny
-// product only lowers the ratio, and the ratio exceeds 1 only when y^2 = h^2
-// exactly, by 2^-23 at most (atan_core is then pi/4 within an ulp).  h = 0
-// gives the ratio 0 and the latitude +-pi/2; y = 0 the latitude +-0.
+// direction with finite components (|x|, |y|, |z| < 1e7), from one
+// r = rsqrt(h^2) (h^2 clamped to [1e-30, 1e30]):
+//   longitude: the angle from the nearer of the z and x axes is
+//     asin(min(|x|, |z|) r), in [0, pi/4]; octant fix-up as atan2_;
+//   latitude: atan(t), t = |y| r, or pi/2 - atan(1/t) above 45 degrees
+//     (t > 1).  The ratio in [0, 1] is min(t, rcp(t)) with no branch: the
+//     reciprocal is below 1 for t > 1 and at least 1 for t < 1, and for t = 0
+//     (rcp_pos gives NaN) the minimum is t (v_min_f32 returns the non-NaN
+//     operand).
+// x = z = 0 gives the longitude +-0 and the latitude +-pi/2 (t = |y| 1e15);
+// y = 0 the latitude +-0.  Below h^2 = 1e-30 (only within 1e-15 of a pole)
+// the clamped r makes the longitude approximate.
 PMP_DF void lonlat(float x, float y, float z, float& lon, float& lat) {
+    const float ax = abs_(x), az = abs_(z);
     const float h2 = fma_(x, x, z * z);
-    const float y2 = y * y;
-    const float rs = rsqrt_pos(med3_(h2 * y2, 1e-30f, 1e30f));
-    const bool big = y2 > h2;
-    float a = atan_core((big ? h2 : y2) * rs);
-    if (big) a = kPiO2 - a;
-    lat = __builtin_copysignf(a, y);
-    lon = atan2_(x, z);
+    const float r = rsqrt_pos(med3_(h2, 1e-30f, 1e30f));
+    float a = asin_oct(med3_(ax, az, 0.0f) * r);
+    if (ax > az) a = kPiO2 - a;
+    if (z < 0.0f) a = kPi - a;
+    lon = __builtin_copysignf(a, x);
+    const float t = abs_(y) * r;
+    float b = atan_core(min_(t, rcp_pos(t)));
+    if (t > 1.0f) b = kPiO2 - b;
+    lat = __builtin_copysignf(b, y);
 }
 
 // exp(x); x <= -87.3 or NaN -> 0; x >= 88 clamped

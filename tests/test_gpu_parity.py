@@ -45,7 +45,7 @@ MATH_CASES = {
 }
 
 
-@pytest.mark.parametrize("name", sorted(MATH_CASES) + ["atan2", "lat"])
+@pytest.mark.parametrize("name", sorted(MATH_CASES) + ["atan2", "lat", "lon"])
 def test_math_bitwise(gpu_ctx, name):
     rng = np.random.default_rng(11)
     if name == "atan2":
@@ -58,6 +58,11 @@ def test_math_bitwise(gpu_ctx, name):
                             [0.0, -0.0, 1.0, -1.0, 1e-20, 7.0]]).astype(np.float32)
         b = np.concatenate([rng.normal(0, 10, 300000), rng.normal(0, 10, 1000),
                             [0.0, 1.0, 0.0, 0.0, 1e-20, 7.0 / np.sqrt(2)]]).astype(np.float32)
+    elif name == "lon":    # longitude of the direction (a, 0, b)
+        a = np.concatenate([rng.normal(0, 10, 300000), rng.uniform(-1, 1, 1000),
+                            [0.0, 0.0, -0.0, 1.0, -1.0, 1e-20, 3.0, -3.0]]).astype(np.float32)
+        b = np.concatenate([rng.normal(0, 10, 300000), rng.uniform(-1e-3, 1e-3, 1000),
+                            [0.0, -1.0, -1.0, 0.0, 0.0, 1e-20, 3.0, -3.0]]).astype(np.float32)
     else:
         a = MATH_CASES[name](rng).astype(np.float32)
         b = None
