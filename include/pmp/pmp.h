@@ -490,12 +490,31 @@ int pmp_write_ply(const char* path, int64_t n, const double* xyz, const uint8_t*
 
 /* Evaluate the device elementary functions (see pmp_math.h) on n inputs:
  * which 0 rcp 1 rsqrt 2 sqrt 3 asin 4 acos 5 atan2(a,b) 6 exp 7 log 8 sin2pi
- * 9 cos2pi 10 latitude of the direction (b, a, b).  Host pointers. */
+ * 9 cos2pi 10 latitude of the direction (b, a, b) 11 longitude of the
+ * direction (a, 0, b).  Host pointers. */
 int pmp_debug_eval_math(pmp_ctx* ctx, int32_t which, int64_t n, const float* a, const float* b, float* out);
 /* The standard normals of refinement for pixel (i, j), iteration it (W wide
  * image, params->seed), n <= 24. */
 int pmp_debug_refine_normals(pmp_ctx* ctx, const pmp_params* params, int32_t width, int32_t i, int32_t j,
                              int32_t it, int32_t n, float* out);
+
+/* Pass epochs of the propagation kernel (race detection for the persistent
+ * work queue; no counterpart in the reference, whose passes are plain
+ * launches, PatchMatchCuda.cu:273-407).  With mode 1 the red-black passes of
+ * pmp_densify and pmp_red_black_pass record per tile how many passes it has
+ * completed, and every tile checks on entry to pass pp that it has completed
+ * pp passes and that the 8 tiles around it (whose maps it reads) have
+ * completed pp or pp + 1; each failed check counts one violation.  Mode 2 also
+ * starts tile 0 one pass ahead, a violation forced from the host (exactly one
+ * in a run that is otherwise correct).  Mode 0 turns it off.  The maps are the
+ * same in every mode. */
+int pmp_debug_epochs(pmp_ctx* ctx, int32_t mode);
+/* The record of the last pass-epoch call: propagation tiles, violations, and
+ * the least and greatest number of passes a tile completed (both equal to
+ * 2 x iterations after a pmp_densify).  PMP_ESTATE before any such call.
+ * Synchronises the context stream. */
+int pmp_debug_epoch_report(pmp_ctx* ctx, int32_t* tiles, int32_t* violations, int32_t* min_epoch,
+                           int32_t* max_epoch);
 
 /* Copy `bytes` (multiple of 16) between two context-owned device buffers with
  * a streaming kernel, `repeats` times on the context stream: a launch of known

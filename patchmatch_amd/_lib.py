@@ -44,7 +44,8 @@ EXPORTS = [
     "pmp_densify", "pmp_set_maps", "pmp_get_maps", "pmp_get_normals", "pmp_eval_counts", "pmp_clean", "pmp_clean_ex",
     "pmp_pruner_add_view", "pmp_pruner_prune", "pmp_pruner_pop_head", "pmp_pruner_reset",
     "pmp_pruner_size", "pmp_pruner_get_map", "pmp_debug_eval_math", "pmp_debug_refine_normals",
-    "pmp_set_timing", "pmp_get_timing", "pmp_set_mask", "pmp_debug_copy",
+    "pmp_set_timing", "pmp_get_timing", "pmp_set_mask", "pmp_debug_copy", "pmp_debug_epochs",
+    "pmp_debug_epoch_report",
     "pmp_batch_densify", "pmp_batch_get_maps", "pmp_batch_set_config", "pmp_batch_get_config",
     "pmp_batch_autotune", "pmp_server_create", "pmp_server_destroy", "pmp_server_last_error",
     "pmp_server_submit", "pmp_server_wait", "pmp_server_fetch", "pmp_server_stats", "pmp_server_pause",
@@ -178,6 +179,8 @@ def load():
         "pmp_get_timing": ([_vp, P(Timing)], c_int),
         "pmp_set_mask": ([_vp, i64], c_int),
         "pmp_debug_copy": ([_vp, i64, i32], c_int),
+        "pmp_debug_epochs": ([_vp, i32], c_int),
+        "pmp_debug_epoch_report": ([_vp, _i32p, _i32p, _i32p, _i32p], c_int),
         "pmp_batch_densify": ([_vp, P(Params), i32, P(Keyframe)], c_int),
         "pmp_batch_get_maps": ([_vp, i32, _f32p, _f32p, _f32p, _i32p], c_int),
         "pmp_batch_set_config": ([_vp, P(BatchConfig)], c_int),
@@ -561,7 +564,7 @@ class Context:
 
     # -- diagnostics
     MATH = {"rcp": 0, "rsqrt": 1, "sqrt": 2, "asin": 3, "acos": 4, "atan2": 5, "exp": 6, "log": 7,
-            "sin2pi": 8, "cos2pi": 9, "lat": 10}
+            "sin2pi": 8, "cos2pi": 9, "lat": 10, "lon": 11}
 
     def eval_math(self, name, a, b=None):
         a = np.ascontiguousarray(a, dtype=np.float32)
@@ -570,6 +573,16 @@ class Context:
         self._chk(self.L.pmp_debug_eval_math(self.h, self.MATH[name], a.size, _p(a, _f32p), _p(bb, _f32p),
                                              _p(out, _f32p)), "pmp_debug_eval_math")
         return out
+
+    def debug_epochs(self, mode: int):
+        """Pass epochs of the red-black passes: 0 off, 1 on, 2 on with tile 0
+        started one pass ahead (one violation forced from the host)."""
+        self._chk(self.L.pmp_debug_epochs(self.h, int(mode)), "pmp_debug_epochs")
+
+    def debug_epoch_report(self) -> dict:
+        v = [ctypes.c_int32(0) for _ in range(4)]
+        self._chk(self.L.pmp_debug_epoch_report(self.h, *[ctypes.byref(x) for x in v]), "pmp_debug_epoch_report")
+        return dict(zip(("tiles", "violations", "min_epoch", "max_epoch"), (x.value for x in v)))
 
     def debug_copy(self, nbytes: int, repeats: int = 1):
         self._chk(self.L.pmp_debug_copy(self.h, nbytes, repeats), "pmp_debug_copy")

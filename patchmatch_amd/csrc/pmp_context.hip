@@ -252,7 +252,8 @@ void pmp_destroy(pmp_ctx* ctx) {
     for (auto& kv : ctx->images) free_image(kv.second);
     void* bufs[] = {ctx->mask, ctx->counters, ctx->thumb.src, ctx->thumb.gray, ctx->thumb.dst, ctx->thumb.taps,
                     ctx->clean_maps, ctx->clean_poses, ctx->normals, ctx->prune_z, ctx->prune_alive, ctx->prune_out,
-                    ctx->prune_pts, ctx->prune_cols, ctx->prune_kps, ctx->prune_blk, ctx->prune_poses, ctx->dbg};
+                    ctx->prune_pts, ctx->prune_cols, ctx->prune_kps, ctx->prune_blk, ctx->prune_poses, ctx->dbg,
+                    ctx->epochs};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     for (auto& kv : ctx->rowtabs) (void)hipFree(kv.second);

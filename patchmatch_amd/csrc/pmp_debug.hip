@@ -2,6 +2,8 @@
 // evaluated on host-provided inputs, for bit-for-bit checks against the host.
 #include <string.h>
 
+#include <vector>
+
 #include "pmp_internal.h"
 #include "pmp_math.h"
 #include "pmp_score.h"
@@ -26,6 +28,7 @@ __global__ void k_eval_math(int which, int n, const float* __restrict__ a, const
         case 8: sincos2pi(x, &s, &c); r = s; break;
         case 9: sincos2pi(x, &s, &c); r = c; break;
         case 10: lonlat(b[k], x, b[k], s, r); break;   // latitude of (b, a, b)
+        case 11: lonlat(x, 0.0f, b[k], r, s); break;  // longitude of (a, 0, b)
         default: break;
     }
     out[k] = r;
@@ -60,7 +63,7 @@ using namespace pmp;
 extern "C" {
 
 int pmp_debug_eval_math(pmp_ctx* ctx, int32_t which, int64_t n, const float* a, const float* b, float* out) {
-    if (!ctx || !a || !out || n < 0 || n > (1ll << 28) || which < 0 || which > 10) return PMP_EINVAL;
+    if (!ctx || !a || !out || n < 0 || n > (1ll << 28) || which < 0 || which > 11) return PMP_EINVAL;
     if (n == 0) return PMP_OK;
     int rc = ensure_dbg(ctx, 3 * (size_t)n);
     if (rc) return rc;
@@ -88,6 +91,34 @@ int pmp_debug_copy(pmp_ctx* ctx, int64_t bytes, int32_t repeats) {
         hipLaunchKernelGGL(k_copy, dim3(4096), dim3(256), 0, ctx->stream, src, dst, n);
     PMP_HIP(ctx, hipGetLastError());
     PMP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return PMP_OK;
+}
+
+int pmp_debug_epochs(pmp_ctx* ctx, int32_t mode) {
+    if (!ctx || mode < 0 || mode > 2) return PMP_EINVAL;
+    ctx->epoch_mode = mode;
+    return PMP_OK;
+}
+
+int pmp_debug_epoch_report(pmp_ctx* ctx, int32_t* tiles, int32_t* violations, int32_t* min_epoch,
+                           int32_t* max_epoch) {
+    if (!ctx || !tiles || !violations || !min_epoch || !max_epoch) return PMP_EINVAL;
+    if (!ctx->epochs || ctx->epoch_tiles <= 0)
+        return set_error(ctx, PMP_ESTATE, "no red-black pass has run with pass epochs on");
+    const int n = ctx->epoch_tiles;
+    std::vector<unsigned int> h((size_t)n + 1);
+    PMP_HIP(ctx, hipMemcpyAsync(h.data(), ctx->epochs, h.size() * sizeof(unsigned int), hipMemcpyDeviceToHost,
+                                ctx->stream));
+    PMP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    unsigned int lo = h[0], hi = h[0];
+    for (int t = 1; t < n; ++t) {
+        lo = h[t] < lo ? h[t] : lo;
+        hi = h[t] > hi ? h[t] : hi;
+    }
+    *tiles = n;
+    *violations = (int32_t)h[(size_t)n];
+    *min_epoch = (int32_t)lo;
+    *max_epoch = (int32_t)hi;
     return PMP_OK;
 }
 

@@ -113,9 +113,11 @@ int build_args(pmp_ctx* ctx, const pmp_params* p, uint64_t ref_id, int N, const 
 
 // The current keyframe as a one-slot descriptor set over the workspace, uploaded.
 // dmask (device, nullable) is the finalize mask; with_mask: use the pooled
-// mask of pmp_set_mask when dmask is null.
+// mask of pmp_set_mask when dmask is null.  propagates: the call runs
+// red-black passes from pass0, so the pass epochs are armed when
+// pmp_debug_epochs has turned them on.
 int plain_prepare(pmp_ctx* ctx, const pmp_params* p, const uint8_t* dmask, bool with_mask, int pass0,
-                         QueueState*& q, QueueRun& r) {
+                  bool propagates, QueueState*& q, QueueRun& r) {
     if (!ctx->views_set) return set_error(ctx, PMP_ESTATE, "pmp_set_views has not been called");
     int rc = queue_state(ctx, q);
     if (rc) return rc;
@@ -141,6 +143,25 @@ int plain_prepare(pmp_ctx* ctx, const pmp_params* p, const uint8_t* dmask, bool 
     fill_list(q->plain.h_hdr->init, 1, &items[0]);
     fill_list(q->plain.h_hdr->prop, 1, &items[1]);
     fill_list(q->plain.h_hdr->rows, 1, &items[2]);
+    if (propagates && ctx->epoch_mode) {
+        // every tile starts at pass0 (mode 2: tile 0 one pass ahead, a
+        // violation forced from the host); fill kernels on the stream, ahead
+        // of the (possibly graph-replayed) passes
+        const size_t need = (size_t)items[1] + 1;
+        if (need > ctx->epochs_cap) {
+            if (ctx->epochs) (void)hipFree(ctx->epochs);
+            ctx->epochs = nullptr;
+            ctx->epochs_cap = 0;
+            PMP_HIP(ctx, hipMalloc(&ctx->epochs, need * sizeof(unsigned int)));
+            ctx->epochs_cap = need;
+        }
+        if ((rc = q_fill_u32(ctx, ctx->stream, ctx->epochs, (size_t)items[1], (unsigned int)pass0))) return rc;
+        if ((rc = q_fill_u32(ctx, ctx->stream, ctx->epochs + items[1], 1, 0u))) return rc;
+        if (ctx->epoch_mode == 2 && (rc = q_fill_u32(ctx, ctx->stream, ctx->epochs, 1, (unsigned int)pass0 + 1u)))
+            return rc;
+        q->plain.h_slots[0].epochs = ctx->epochs;
+        ctx->epoch_tiles = items[1];
+    }
     if ((rc = queue_run(ctx, *q, q->cfg, p->edge_mode == PMP_EDGE_ZERO, p->postprocess != 0, r))) return rc;
     if (p->iterations > kMaxPasses / 2) return set_error(ctx, PMP_EINVAL, "iterations must be <= 64");
     return desc_upload(ctx, q->plain, 1, ctx->stream);
@@ -226,7 +247,7 @@ int pmp_random_initialization(pmp_ctx* ctx, const pmp_params* params) {
     if (!ctx || !params) return PMP_EINVAL;
     QueueState* q = nullptr;
     QueueRun r;
-    int rc = plain_prepare(ctx, params, nullptr, false, 0, q, r);
+    int rc = plain_prepare(ctx, params, nullptr, false, 0, false, q, r);
     if (rc) return rc;
     const int tp = timing_begin(ctx, kStageInit);
     rc = q_init(ctx, ctx->stream, q->plain, r);
@@ -238,7 +259,7 @@ int pmp_score_plane_depth(pmp_ctx* ctx, const pmp_params* params) {
     if (!ctx || !params) return PMP_EINVAL;
     QueueState* q = nullptr;
     QueueRun r;
-    int rc = plain_prepare(ctx, params, nullptr, false, 0, q, r);
+    int rc = plain_prepare(ctx, params, nullptr, false, 0, false, q, r);
     if (rc) return rc;
     return q_score(ctx, ctx->stream, q->plain, r);
 }
@@ -248,7 +269,7 @@ int pmp_red_black_pass(pmp_ctx* ctx, const pmp_params* params, int32_t it_begin,
     if (it_end - it_begin > kMaxPasses / 2) return set_error(ctx, PMP_EINVAL, "at most 64 iterations per call");
     QueueState* q = nullptr;
     QueueRun r;
-    int rc = plain_prepare(ctx, params, nullptr, false, 2 * it_begin, q, r);
+    int rc = plain_prepare(ctx, params, nullptr, false, 2 * it_begin, true, q, r);
     if (rc) return rc;
     const int passes = 2 * (it_end - it_begin);
     if ((rc = q_clear(ctx, ctx->stream, q->plain, passes))) return rc;
@@ -269,7 +290,7 @@ int pmp_finalize(pmp_ctx* ctx, const pmp_params* params, const uint8_t* mask) {
     if (rc) return rc;
     QueueState* q = nullptr;
     QueueRun r;
-    if ((rc = plain_prepare(ctx, params, dmask, true, 0, q, r))) return rc;
+    if ((rc = plain_prepare(ctx, params, dmask, true, 0, false, q, r))) return rc;
     const int tp = timing_begin(ctx, kStageFinalize);
     rc = q_finalize(ctx, ctx->stream, q->plain, r);
     timing_end(ctx, tp);
@@ -284,7 +305,7 @@ int pmp_densify(pmp_ctx* ctx, const pmp_params* params, const uint8_t* mask) {
     if (rc) return rc;
     QueueState* q = nullptr;
     QueueRun r;
-    if ((rc = plain_prepare(ctx, params, dmask, true, 0, q, r))) return rc;
+    if ((rc = plain_prepare(ctx, params, dmask, true, 0, true, q, r))) return rc;
     if ((rc = q_densify(ctx, ctx->stream, q->plain, r, 2 * params->iterations, q->cfg.use_graph != 0))) return rc;
     ctx->densify_calls++;
     return PMP_OK;

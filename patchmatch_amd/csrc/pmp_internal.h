@@ -158,6 +158,13 @@ struct pmp_ctx {
     // debug
     float* dbg = nullptr;
     size_t dbg_cap = 0;
+    // pass epochs of the one-keyframe propagation (pmp_debug_epochs): mode,
+    // device array (tiles + 1 words), its capacity, and the last call's
+    // tile count and expected final epoch
+    int epoch_mode = 0;
+    unsigned int* epochs = nullptr;
+    size_t epochs_cap = 0;
+    int epoch_tiles = 0, epoch_final = 0;
 };
 
 namespace pmp {

@@ -341,7 +341,7 @@ int pmp_preinit_landmarks(pmp_ctx* ctx, const pmp_params* params, int64_t n, con
     int rc = stage_mask(ctx, mask, dmask);
     QueueState* q = nullptr;
     QueueRun r;
-    if (!rc) rc = plain_prepare(ctx, params, dmask, true, 0, q, r);
+    if (!rc) rc = plain_prepare(ctx, params, dmask, true, 0, false, q, r);
     PreState* s = nullptr;
     if (!rc) rc = pre_state(ctx, s);
     if (!rc) rc = ensure_cands(ctx, s, (size_t)n, (size_t)ctx->W * ctx->H);
@@ -395,7 +395,7 @@ int pmp_preinit_warp(pmp_ctx* ctx, const pmp_params* params, const uint8_t* mask
     int rc = stage_mask(ctx, mask, dmask);
     QueueState* q = nullptr;
     QueueRun r;
-    if (!rc) rc = plain_prepare(ctx, params, dmask, true, 0, q, r);
+    if (!rc) rc = plain_prepare(ctx, params, dmask, true, 0, false, q, r);
     const size_t P = (size_t)ctx->W * ctx->H;
     if (!rc) rc = ensure_cands(ctx, s, P, P);
     if (rc) return rc;

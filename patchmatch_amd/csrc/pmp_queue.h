@@ -40,6 +40,10 @@ struct SlotDesc {
     int pass0;                // red-black pass of this slot at launch pass 0
     int nbx_init;             // 64-pixel tiles per row
     int nbx_prop;             // propagation tiles per tile row (kTileSame same-colour pixels each)
+    int nby_prop;             // propagation tile rows
+    // nullable (debug): per propagation tile, the number of red-black passes
+    // it has completed; epochs[nbx_prop * nby_prop] counts violations
+    unsigned int* epochs;
 };
 
 // Items of one stage: the slots taking part and the prefix sums of their
@@ -119,7 +123,7 @@ int q_fill_u32(pmp_ctx* ctx, hipStream_t s, unsigned int* p, size_t n, unsigned 
 // The one-keyframe API's current keyframe as a one-slot set over the context
 // workspace, uploaded (pmp_estimator.hip).  dmask (device, nullable) is the
 // mask; with_mask: use the pmp_set_mask image when dmask is null.
-int plain_prepare(pmp_ctx* ctx, const pmp_params* p, const uint8_t* dmask, bool with_mask, int pass0,
+int plain_prepare(pmp_ctx* ctx, const pmp_params* p, const uint8_t* dmask, bool with_mask, int pass0, bool propagates,
                   QueueState*& q, QueueRun& r);
 // A host mask (nullable) copied to the workspace's staging buffer.
 int stage_mask(pmp_ctx* ctx, const uint8_t* mask_host, const uint8_t*& dmask);

@@ -83,6 +83,40 @@ __global__ __launch_bounds__(256) void k_init_q(const SlotDesc PMP_CONST* slots,
     }
 }
 
+// Debug pass epochs (SlotDesc::epochs).  A tile about to run red-black pass
+// pp must have completed pp passes, and each of the 8 tiles around it (whose
+// maps its candidates and the other colour's pixels come from, 3 pixels at
+// most across a tile edge) pp or pp + 1: the previous pass wrote them and this
+// one writes only the colour that pass pp does not read.  Anything else (a
+// tile skipped, run twice, or run before the pass before it finished) adds to
+// the violation count.  Tiles of one launch may see each other's stores or not;
+// both values are allowed.
+__device__ __forceinline__ void epoch_enter(const SlotDesc& d, int tx, int ty, int pp) {
+    if (threadIdx.x != 0) return;
+    unsigned int* e = d.epochs;
+    const int nbx = d.nbx_prop, nby = d.nby_prop;
+    unsigned int bad = 0;
+    for (int dy = -1; dy <= 1; ++dy)
+        for (int dx = -1; dx <= 1; ++dx) {
+            const int x = tx + dx, y = ty + dy;
+            if (x < 0 || y < 0 || x >= nbx || y >= nby) continue;
+            const unsigned int v = __hip_atomic_load(&e[y * nbx + x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const bool ok = (dx == 0 && dy == 0) ? v == (unsigned int)pp : (v == (unsigned int)pp || v == (unsigned int)pp + 1u);
+            bad += ok ? 0u : 1u;
+        }
+    if (bad) atomicAdd(&e[nbx * nby], bad);
+}
+
+// After every lane's map writes of the tile: its epoch becomes pp + 1.
+__device__ __forceinline__ void epoch_leave(const SlotDesc& d, int tx, int ty, int pp) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        __hip_atomic_store(&d.epochs[ty * d.nbx_prop + tx], (unsigned int)pp + 1u, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
 // The zero-edge variant is held to 4 waves per SIMD (<= 128 VGPRs; left
 // alone it takes 132 and runs at 3 waves, 28 % slower); the wrap/clamp
 // variant fits by itself and is faster with the compiler's own allocation.
@@ -118,8 +152,11 @@ __global__ __launch_bounds__(64 * R) __attribute__((amdgpu_waves_per_eu(kZero ? 
                 cur = s;
             }
             const int pp = d.pass0 + pass;
-            propagate_tile<0, kZero, R>(d.a, sh, d.plane, d.depth, d.score, pp >> 1, pp & 1, local % d.nbx_prop,
-                                        local / d.nbx_prop, 64 * (item & (R - 1)), n_eval, n_cand);
+            const int tx = local % d.nbx_prop, ty = local / d.nbx_prop;
+            if (d.epochs) epoch_enter(d, tx, ty, pp);
+            propagate_tile<0, kZero, R>(d.a, sh, d.plane, d.depth, d.score, pp >> 1, pp & 1, tx, ty,
+                                        64 * (item & (R - 1)), n_eval, n_cand);
+            if (d.epochs) epoch_leave(d, tx, ty, pp);
         }
     }
     count_add(&counters[0], n_eval);
@@ -279,7 +316,8 @@ int slot_fill(pmp_ctx* ctx, const pmp_params* params, const pmp_keyframe& kf, co
     d.nbx_prop = ((W + 1) / 2 + kTileSame - 1) / kTileSame;
     items[0] = d.nbx_init * ((H + 3) / 4);
     const int trows = 64 * rows / kTileSame;   // rows of a propagation tile
-    items[1] = d.nbx_prop * ((H + trows - 1) / trows);
+    d.nby_prop = (H + trows - 1) / trows;
+    items[1] = d.nbx_prop * d.nby_prop;
     items[2] = H;
     return PMP_OK;
 }
