@@ -95,8 +95,15 @@ class Scene:
         obj = face * 2 + (D[np.arange(n), face] > 0)          # 0..5 walls
         for s, (cx, cy, cz, r) in enumerate(self.spheres):
             oc = C - np.array([cx, cy, cz])
-            b = D @ oc if oc.ndim == 1 else np.einsum("ij,ij->i", D, oc)
-            c = float(oc @ oc) - r * r if oc.ndim == 1 else np.einsum("ij,ij->i", oc, oc) - r * r
+            # elementwise products, not BLAS: render() runs on several
+            # threads at once, and the BLAS of this image is not safe for
+            # concurrent calls (its results then vary from run to run)
+            if oc.ndim == 1:
+                b = D[:, 0] * oc[0] + D[:, 1] * oc[1] + D[:, 2] * oc[2]
+                c = float(oc[0] * oc[0] + oc[1] * oc[1] + oc[2] * oc[2]) - r * r
+            else:
+                b = np.einsum("ij,ij->i", D, oc)
+                c = np.einsum("ij,ij->i", oc, oc) - r * r
             disc = b * b - c
             ok = disc > 0
             sq = np.sqrt(np.where(ok, disc, 0.0))
@@ -152,9 +159,10 @@ def render(scene: Scene, pose: np.ndarray, W: int, H: int):
     """Noise-free intensity (float, H x W) and depth (H x W) of one camera."""
     R = pose[:9].reshape(3, 3)
     t = pose[9:]
-    C = -R.T @ t
+    # elementwise, not BLAS (see Scene.cast): C = -R^T t, rows of D = R^T b
+    C = -(R[0] * t[0] + R[1] * t[1] + R[2] * t[2])
     B = bearings(W, H).reshape(-1, 3)
-    D = B @ R            # rows: R^T b
+    D = B[:, :1] * R[0] + B[:, 1:2] * R[1] + B[:, 2:3] * R[2]
     depth, obj = scene.cast(C, D)
     P = C + depth[:, None] * D
     img = scene.texture(P, obj)
