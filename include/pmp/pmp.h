@@ -501,7 +501,8 @@ int pmp_debug_refine_normals(pmp_ctx* ctx, const pmp_params* params, int32_t wid
 /* Pass epochs of the propagation kernel (race detection for the persistent
  * work queue; no counterpart in the reference, whose passes are plain
  * launches, PatchMatchCuda.cu:273-407).  With mode 1 the red-black passes of
- * pmp_densify and pmp_red_black_pass record per tile how many passes it has
+ * pmp_densify, pmp_red_black_pass and pmp_batch_densify (every keyframe of
+ * the batch; not its autotuner) record per tile how many passes it has
  * completed, and every tile checks on entry to pass pp that it has completed
  * pp passes and that the 8 tiles around it (whose maps it reads) have
  * completed pp or pp + 1; each failed check counts one violation.  Mode 2 also
@@ -509,9 +510,10 @@ int pmp_debug_refine_normals(pmp_ctx* ctx, const pmp_params* params, int32_t wid
  * in a run that is otherwise correct).  Mode 0 turns it off.  The maps are the
  * same in every mode. */
 int pmp_debug_epochs(pmp_ctx* ctx, int32_t mode);
-/* The record of the last pass-epoch call: propagation tiles, violations, and
- * the least and greatest number of passes a tile completed (both equal to
- * 2 x iterations after a pmp_densify).  PMP_ESTATE before any such call.
+/* The record of the last pass-epoch call: propagation tiles (over every
+ * keyframe of a batch), violations, and the least and greatest number of
+ * passes a tile completed (both equal to 2 x iterations after a pmp_densify
+ * or pmp_batch_densify).  PMP_ESTATE before any such call.
  * Synchronises the context stream. */
 int pmp_debug_epoch_report(pmp_ctx* ctx, int32_t* tiles, int32_t* violations, int32_t* min_epoch,
                            int32_t* max_epoch);

@@ -103,20 +103,28 @@ int pmp_debug_epochs(pmp_ctx* ctx, int32_t mode) {
 int pmp_debug_epoch_report(pmp_ctx* ctx, int32_t* tiles, int32_t* violations, int32_t* min_epoch,
                            int32_t* max_epoch) {
     if (!ctx || !tiles || !violations || !min_epoch || !max_epoch) return PMP_EINVAL;
-    if (!ctx->epochs || ctx->epoch_tiles <= 0)
+    if (!ctx->epochs || ctx->epoch_tiles.empty())
         return set_error(ctx, PMP_ESTATE, "no red-black pass has run with pass epochs on");
-    const int n = ctx->epoch_tiles;
-    std::vector<unsigned int> h((size_t)n + 1);
+    std::vector<unsigned int> h(ctx->epoch_words);
     PMP_HIP(ctx, hipMemcpyAsync(h.data(), ctx->epochs, h.size() * sizeof(unsigned int), hipMemcpyDeviceToHost,
                                 ctx->stream));
     PMP_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    unsigned int lo = h[0], hi = h[0];
-    for (int t = 1; t < n; ++t) {
-        lo = h[t] < lo ? h[t] : lo;
-        hi = h[t] > hi ? h[t] : hi;
+    // segments in slot order: the slot's tiles, then its violation count
+    unsigned int lo = ~0u, hi = 0;
+    long long n_tiles = 0, bad = 0;
+    size_t off = 0;
+    for (const int t : ctx->epoch_tiles) {
+        for (int i = 0; i < t; ++i) {
+            const unsigned int v = h[off + (size_t)i];
+            lo = v < lo ? v : lo;
+            hi = v > hi ? v : hi;
+        }
+        bad += h[off + (size_t)t];
+        n_tiles += t;
+        off += (size_t)t + 1;
     }
-    *tiles = n;
-    *violations = (int32_t)h[(size_t)n];
+    *tiles = (int32_t)n_tiles;
+    *violations = (int32_t)bad;
     *min_epoch = (int32_t)lo;
     *max_epoch = (int32_t)hi;
     return PMP_OK;

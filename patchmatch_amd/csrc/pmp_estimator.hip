@@ -143,25 +143,7 @@ int plain_prepare(pmp_ctx* ctx, const pmp_params* p, const uint8_t* dmask, bool 
     fill_list(q->plain.h_hdr->init, 1, &items[0]);
     fill_list(q->plain.h_hdr->prop, 1, &items[1]);
     fill_list(q->plain.h_hdr->rows, 1, &items[2]);
-    if (propagates && ctx->epoch_mode) {
-        // every tile starts at pass0 (mode 2: tile 0 one pass ahead, a
-        // violation forced from the host); fill kernels on the stream, ahead
-        // of the (possibly graph-replayed) passes
-        const size_t need = (size_t)items[1] + 1;
-        if (need > ctx->epochs_cap) {
-            if (ctx->epochs) (void)hipFree(ctx->epochs);
-            ctx->epochs = nullptr;
-            ctx->epochs_cap = 0;
-            PMP_HIP(ctx, hipMalloc(&ctx->epochs, need * sizeof(unsigned int)));
-            ctx->epochs_cap = need;
-        }
-        if ((rc = q_fill_u32(ctx, ctx->stream, ctx->epochs, (size_t)items[1], (unsigned int)pass0))) return rc;
-        if ((rc = q_fill_u32(ctx, ctx->stream, ctx->epochs + items[1], 1, 0u))) return rc;
-        if (ctx->epoch_mode == 2 && (rc = q_fill_u32(ctx, ctx->stream, ctx->epochs, 1, (unsigned int)pass0 + 1u)))
-            return rc;
-        q->plain.h_slots[0].epochs = ctx->epochs;
-        ctx->epoch_tiles = items[1];
-    }
+    if (propagates && (rc = epochs_arm(ctx, ctx->stream, 1, &q->plain.h_slots[0], &items[1], pass0))) return rc;
     if ((rc = queue_run(ctx, *q, q->cfg, p->edge_mode == PMP_EDGE_ZERO, p->postprocess != 0, r))) return rc;
     if (p->iterations > kMaxPasses / 2) return set_error(ctx, PMP_EINVAL, "iterations must be <= 64");
     return desc_upload(ctx, q->plain, 1, ctx->stream);

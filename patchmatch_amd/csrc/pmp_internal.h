@@ -158,13 +158,14 @@ struct pmp_ctx {
     // debug
     float* dbg = nullptr;
     size_t dbg_cap = 0;
-    // pass epochs of the one-keyframe propagation (pmp_debug_epochs): mode,
-    // device array (tiles + 1 words), its capacity, and the last call's
-    // tile count and expected final epoch
+    // pass epochs of the propagation (pmp_debug_epochs): mode, device array
+    // (per slot of the last armed call: its tiles' epochs, then its violation
+    // count), its capacity, and that call's tiles per slot and total words
     int epoch_mode = 0;
     unsigned int* epochs = nullptr;
     size_t epochs_cap = 0;
-    int epoch_tiles = 0, epoch_final = 0;
+    std::vector<int> epoch_tiles;
+    size_t epoch_words = 0;
 };
 
 namespace pmp {

@@ -119,6 +119,11 @@ int q_finalize(pmp_ctx* ctx, hipStream_t s, DescSet& d, const QueueRun& r);
 int q_densify(pmp_ctx* ctx, hipStream_t s, DescSet& d, const QueueRun& r, int passes, bool use_graph);
 // n words of p set to v with a fill kernel (capture-safe).
 int q_fill_u32(pmp_ctx* ctx, hipStream_t s, unsigned int* p, size_t n, unsigned int v);
+// With pmp_debug_epochs on: gives each of the n slots a segment of the
+// context's epoch array (tiles[k] epochs set to pass0, then a zero violation
+// count; mode 2 starts slot 0's tile 0 one pass ahead), on stream s, and
+// points slots[k].epochs at it.  Off: leaves the slots' epochs null.
+int epochs_arm(pmp_ctx* ctx, hipStream_t s, int n, SlotDesc* slots, const int* tiles, int pass0);
 
 // The one-keyframe API's current keyframe as a one-slot set over the context
 // workspace, uploaded (pmp_estimator.hip).  dmask (device, nullable) is the

@@ -436,6 +436,33 @@ int q_fill_u32(pmp_ctx* ctx, hipStream_t s, unsigned int* p, size_t n, unsigned 
     return PMP_OK;
 }
 
+int epochs_arm(pmp_ctx* ctx, hipStream_t s, int n, SlotDesc* slots, const int* tiles, int pass0) {
+    if (!ctx->epoch_mode) return PMP_OK;
+    size_t need = 0;
+    for (int k = 0; k < n; ++k) need += (size_t)tiles[k] + 1;
+    if (need > ctx->epochs_cap) {
+        if (ctx->epochs) (void)hipFree(ctx->epochs);
+        ctx->epochs = nullptr;
+        ctx->epochs_cap = 0;
+        PMP_HIP(ctx, hipMalloc(&ctx->epochs, need * sizeof(unsigned int)));
+        ctx->epochs_cap = need;
+    }
+    // fill kernels on the stream, ahead of the (possibly graph-replayed) passes
+    size_t off = 0;
+    int rc;
+    for (int k = 0; k < n; ++k) {
+        unsigned int* seg = ctx->epochs + off;
+        if ((rc = q_fill_u32(ctx, s, seg, (size_t)tiles[k], (unsigned int)pass0))) return rc;
+        if ((rc = q_fill_u32(ctx, s, seg + tiles[k], 1, 0u))) return rc;
+        if (k == 0 && ctx->epoch_mode == 2 && (rc = q_fill_u32(ctx, s, seg, 1, (unsigned int)pass0 + 1u))) return rc;
+        slots[k].epochs = seg;
+        off += (size_t)tiles[k] + 1;
+    }
+    ctx->epoch_tiles.assign(tiles, tiles + n);
+    ctx->epoch_words = need;
+    return PMP_OK;
+}
+
 static int q_sequence(pmp_ctx* ctx, hipStream_t s, DescSet& d, const QueueRun& r, int passes) {
     int rc;
     if ((rc = q_clear(ctx, s, d, passes))) return rc;
@@ -484,8 +511,10 @@ int q_densify(pmp_ctx* ctx, hipStream_t s, DescSet& d, const QueueRun& r, int pa
 
 // Descriptors and work lists for n keyframes (all from pass 0), uploaded to
 // the batch set.
+// arm: pass epochs for this batch (pmp_batch_densify; the autotuner reruns
+// passes from the same maps and does not arm them)
 static int batch_prepare(pmp_ctx* ctx, QueueState* q, const pmp_params* params, int n, const pmp_keyframe* kfs,
-                         int rows) {
+                         int rows, bool arm) {
     int rc = check_params(ctx, params);
     if (rc) return rc;
     if (params->iterations > kMaxPasses / 2) return set_error(ctx, PMP_EINVAL, "iterations must be <= 64");
@@ -511,6 +540,7 @@ static int batch_prepare(pmp_ctx* ctx, QueueState* q, const pmp_params* params, 
     fill_list(q->batch.h_hdr->init, n, it_init);
     fill_list(q->batch.h_hdr->prop, n, it_prop);
     fill_list(q->batch.h_hdr->rows, n, it_rows);
+    if (arm && (rc = epochs_arm(ctx, ctx->stream, n, q->batch.h_slots, it_prop, 0))) return rc;
     return desc_upload(ctx, q->batch, n, ctx->stream);
 }
 
@@ -528,7 +558,7 @@ int pmp_batch_densify(pmp_ctx* ctx, const pmp_params* params, int32_t n, const p
     int rc = queue_state(ctx, q);
     if (rc) return rc;
     q->n_last = 0;
-    if ((rc = batch_prepare(ctx, q, params, n, kfs, q->cfg.tile_rows))) return rc;
+    if ((rc = batch_prepare(ctx, q, params, n, kfs, q->cfg.tile_rows, true))) return rc;
     QueueRun r;
     if ((rc = queue_run(ctx, *q, q->cfg, params->edge_mode == PMP_EDGE_ZERO, params->postprocess != 0, r))) return rc;
     if ((rc = q_densify(ctx, ctx->stream, q->batch, r, 2 * params->iterations, q->cfg.use_graph != 0))) return rc;
@@ -587,7 +617,7 @@ int pmp_batch_autotune(pmp_ctx* ctx, const pmp_params* params, int32_t n, const 
     const int rows_set[3] = {2, 4, 8};
     const bool zero = params->edge_mode == PMP_EDGE_ZERO;
     for (int ri = 0; ri < 3 && rc == 0; ++ri) {
-        if ((rc = batch_prepare(ctx, q, params, n, kfs, rows_set[ri]))) break;
+        if ((rc = batch_prepare(ctx, q, params, n, kfs, rows_set[ri], false))) break;
         pmp_batch_config c = q->cfg;
         c.tile_rows = rows_set[ri];
         c.blocks_per_cu = 32;
