@@ -151,6 +151,8 @@ def main(argv=None):
     ap.add_argument("--rates", type=str, default="5,15,30", help="served arrival rates (requests/s)")
     ap.add_argument("--pipeline", type=int, default=30,
                     help="keyframes of the densify -> clean -> fuse trajectory at the plain size (0: skip)")
+    ap.add_argument("--params", type=str, default="",
+                    help="JSON params config for every leg (params.params_from_json; default: config 1 with --seed)")
     args = ap.parse_args(argv)
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -159,7 +161,7 @@ def main(argv=None):
         return 2
 
     from patchmatch_amd import synth
-    from patchmatch_amd.params import config1_params
+    from patchmatch_amd.params import config1_params, params_from_json
 
     t0 = time.perf_counter()
     kf = synth.config1(args.seed, W=args.width, H=args.height, n_views=args.views)   # before any GPU use
@@ -168,7 +170,7 @@ def main(argv=None):
           flush=True)
 
     from patchmatch_amd._lib import PMP_MAPS_FINAL, PMP_MAPS_INIT, Context
-    p = config1_params(seed=args.seed)
+    p = params_from_json(args.params) if args.params else config1_params(seed=args.seed)
     ctx = Context(0)
     ctx.load_keyframe(kf.ref, kf.views, kf.ref_pose, kf.view_poses)
     mask_id = 1000
@@ -238,10 +240,12 @@ def main(argv=None):
         "dtype": "fp32",
         "data": "synthetic (seeded ray-cast scene, rendered in-process)",
         "config": {"workload": "plain drop-in: 1 keyframe 1920x960 + 6 neighbours, 11x11 patch stride 2, "
-                               "best-3-of-6 NCC, 8 red-black iterations" if (args.width, args.height, N) == (1920, 960, 6)
-                   else f"plain drop-in: 1 keyframe {args.width}x{args.height} + {N} neighbours",
+                               "best-3-of-6 NCC, 8 red-black iterations"
+                   if (args.width, args.height, N) == (1920, 960, 6) and not args.params
+                   else f"plain drop-in: 1 keyframe {args.width}x{args.height} + {N} neighbours"
+                        + (f", params from {args.params}" if args.params else ""),
                    "width": args.width, "height": args.height, "views": N, "taps": T, "iterations": p.iterations,
-                   "form": "plain"},
+                   "form": "plain", **({"params": p.as_dict()} if args.params else {})},
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3), "p99": round(float(np.percentile(lat, 99)), 3),
                        "max": round(float(np.max(lat)), 3)},
         "mpx_iter_per_s": round(mpx_iter * value, 2),

@@ -85,3 +85,60 @@ def config1_params(**kw) -> Params:
              init_mode=INIT_INVDEPTH_HEMI, coverage_mode=COVER_FULL, max_depth=20.0, seed=0)
     d.update(kw)
     return default_params(**d)
+
+
+# ---- JSON configs for the tools (bench.py --params, profile_run --params)
+
+CONFIGS = {"default": default_params, "config0": config0_params, "config1": config1_params}
+ENUMS = {
+    "agg_mode": {"ref_fold": AGG_REF_FOLD, "max": AGG_MAX, "topk_mean": AGG_TOPK_MEAN, "fold_indep": AGG_FOLD_INDEP},
+    "edge_mode": {"wrap": EDGE_WRAP, "clamp": EDGE_CLAMP, "zero": EDGE_ZERO},
+    "coverage_mode": {"ref_grid": COVER_REF_GRID, "full": COVER_FULL},
+    "init_mode": {"ref_uniform": INIT_REF_UNIFORM, "invdepth_hemi": INIT_INVDEPTH_HEMI},
+}
+
+
+def params_from_json(src) -> Params:
+    """Params from a JSON config: a path, a JSON text or a dict such as
+    {"base": "config1", "iterations": 4, "agg_mode": "max"}.  "base" names
+    the starting values (default, config0, config1; default config1); every
+    other key is a pmp_params field, with enum fields given by value or by
+    name.  Unknown keys and values of the wrong type raise ValueError (the
+    library checks ranges when the params are used)."""
+    import json
+    import os
+    if isinstance(src, dict):
+        d = dict(src)
+    elif isinstance(src, str) and src.lstrip().startswith(("{", "[")):
+        d = json.loads(src)
+    else:
+        if not os.path.exists(src):
+            raise ValueError(f"no such params file: {src}")
+        with open(src) as f:
+            d = json.load(f)
+    if not isinstance(d, dict):
+        raise ValueError("a params config is a JSON object")
+    base = d.pop("base", "config1")
+    if base not in CONFIGS:
+        raise ValueError(f"unknown base {base!r} (one of {sorted(CONFIGS)})")
+    types = dict(Params._fields_)
+    kw = {}
+    for k, v in d.items():
+        if k not in types:
+            raise ValueError(f"unknown params field {k!r}")
+        if k in ENUMS and isinstance(v, str):
+            if v not in ENUMS[k]:
+                raise ValueError(f"{k}: unknown value {v!r} (one of {sorted(ENUMS[k])})")
+            v = ENUMS[k][v]
+        floating = types[k] in (ctypes.c_float, ctypes.c_double)
+        if isinstance(v, bool) or not isinstance(v, (int, float)) or (not floating and not isinstance(v, int)):
+            raise ValueError(f"{k}: {'a number' if floating else 'an integer'} expected, got {v!r}")
+        kw[k] = v
+    return CONFIGS[base](**kw)
+
+
+def params_to_json(p: Params) -> str:
+    """Every field of p as a JSON object (base "default", so that
+    params_from_json gives p back)."""
+    import json
+    return json.dumps({"base": "default", **p.as_dict()}, indent=1)

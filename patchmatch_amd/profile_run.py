@@ -7,6 +7,7 @@ hardware byte counters: a copy of B bytes moves B bytes in and B bytes out).
     python -m patchmatch_amd.profile_run --steps 2 --copy-mib 256
     python -m patchmatch_amd.profile_run --steps 1 --pipeline 10 --copy-mib 256
     python -m patchmatch_amd.profile_run --steps 0 --batched 64 --copy-mib 0
+    python -m patchmatch_amd.profile_run --steps 1 --params '{"edge_mode": "zero"}'
 
 --batched n runs the batched workload of the benchmark (n keyframes of mixed
 resolution and view count, plain then batched, workloads.run_batched).
@@ -32,15 +33,16 @@ def main(argv=None):
     ap.add_argument("--views", type=int, default=6)
     ap.add_argument("--pipeline", type=int, default=0, help="keyframes of the densify -> clean -> fuse trajectory")
     ap.add_argument("--batched", type=int, default=0, help="keyframes of the batched workload")
+    ap.add_argument("--params", type=str, default="", help="JSON params config (params.params_from_json)")
     args = ap.parse_args(argv)
 
     from patchmatch_amd import synth
-    from patchmatch_amd.params import config1_params
+    from patchmatch_amd.params import config1_params, params_from_json
     kf = synth.config1(0, W=args.width, H=args.height, n_views=args.views)
     print(f"[profile_run] rendered {args.width}x{args.height}+{args.views}", flush=True)
 
     from patchmatch_amd._lib import Context
-    p = config1_params()
+    p = params_from_json(args.params) if args.params else config1_params()
     with Context(0) as ctx:
         ctx.load_keyframe(kf.ref, kf.views, kf.ref_pose, kf.view_poses)
         ctx.image_put(1000, kf.mask)
