@@ -288,6 +288,10 @@ def main(argv=None):
 
     if args.cpu_seconds > 0:
         threads = args.cpu_threads or min(16, len(os.sched_getaffinity(0)))
+        # OpenMP reads these when the oracle (and with it libgomp) is first
+        # loaded, which is here: threads bound to cores, close together
+        os.environ.setdefault("OMP_PROC_BIND", "close")
+        os.environ.setdefault("OMP_PLACES", "cores")
         init_maps = ctx.get_maps(PMP_MAPS_INIT)
         eps, sample = cpu_baseline(kf, p, init_maps, args.cpu_seconds, threads)
         init_evals = P_int
@@ -300,6 +304,9 @@ def main(argv=None):
                        f"{sample['seconds']:.1f} s); depth maps/s = evaluations/s / {int(evals_per_keyframe)} "
                        f"evaluations per keyframe (init + passes, as counted on the GPU)"),
             "evals_per_s": round(eps, 1),
+            "tne_per_s": round(eps * T * N, 1),
+            "mpx_iter_per_s": round(mpx_iter * cpu_value, 4),
+            "omp": {"proc_bind": os.environ["OMP_PROC_BIND"], "places": os.environ["OMP_PLACES"]},
             "cpu_model": cpu_model(), "nproc": os.cpu_count(),
             "c0_end_to_end_s": round(cpu_c0_end_to_end(threads), 3),
             "c0_gpu_end_to_end_ms": round(gpu_c0_end_to_end(ctx), 3),
