@@ -518,6 +518,16 @@ int pmp_debug_epochs(pmp_ctx* ctx, int32_t mode);
 int pmp_debug_epoch_report(pmp_ctx* ctx, int32_t* tiles, int32_t* violations, int32_t* min_epoch,
                            int32_t* max_epoch);
 
+/* roctx ranges around the main calls (pmp_densify and the other stages,
+ * pmp_batch_*, pmp_clean, pmp_pruner_prune, pmp_preinit_*, pmp_image_put, and
+ * the served form's uploads and steps), for rocprofv3 --marker-trace: host
+ * ranges of the enqueue, beside the kernel trace.  The reference brackets its
+ * wrappers with cudaEvent timings printed to stdout instead
+ * (PatchMatchCuda.cu:421-441).  Process-wide; also turned on by the
+ * environment variable PMP_ROCTX=1.  The ROCm marker library is opened at run
+ * time: PMP_ENOTFOUND when it cannot be, and tracing stays off. */
+int pmp_set_roctx(int32_t on);
+
 /* Copy `bytes` (multiple of 16) between two context-owned device buffers with
  * a streaming kernel, `repeats` times on the context stream: a launch of known
  * traffic for calibrating hardware byte counters. */

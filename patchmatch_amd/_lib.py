@@ -45,7 +45,7 @@ EXPORTS = [
     "pmp_pruner_add_view", "pmp_pruner_prune", "pmp_pruner_pop_head", "pmp_pruner_reset",
     "pmp_pruner_size", "pmp_pruner_get_map", "pmp_debug_eval_math", "pmp_debug_refine_normals",
     "pmp_set_timing", "pmp_get_timing", "pmp_set_mask", "pmp_debug_copy", "pmp_debug_epochs",
-    "pmp_debug_epoch_report",
+    "pmp_debug_epoch_report", "pmp_set_roctx",
     "pmp_batch_densify", "pmp_batch_get_maps", "pmp_batch_set_config", "pmp_batch_get_config",
     "pmp_batch_autotune", "pmp_server_create", "pmp_server_destroy", "pmp_server_last_error",
     "pmp_server_submit", "pmp_server_wait", "pmp_server_fetch", "pmp_server_stats", "pmp_server_pause",
@@ -180,6 +180,7 @@ def load():
         "pmp_set_mask": ([_vp, i64], c_int),
         "pmp_debug_copy": ([_vp, i64, i32], c_int),
         "pmp_debug_epochs": ([_vp, i32], c_int),
+        "pmp_set_roctx": ([i32], c_int),
         "pmp_debug_epoch_report": ([_vp, _i32p, _i32p, _i32p, _i32p], c_int),
         "pmp_batch_densify": ([_vp, P(Params), i32, P(Keyframe)], c_int),
         "pmp_batch_get_maps": ([_vp, i32, _f32p, _f32p, _f32p, _i32p], c_int),
@@ -270,6 +271,12 @@ def write_dense_msgpack(path: str, camera: str, cols: int, rows: int, keyframes,
     if rc != 0:
         raise PmpError(f"pmp_write_dense_msgpack: {STATUS.get(rc, rc)}")
 
+
+
+def set_roctx(on: bool) -> bool:
+    """roctx ranges around the library's main calls (process-wide); False if
+    the ROCm marker library could not be opened."""
+    return load().pmp_set_roctx(1 if on else 0) == 0
 
 class Context:
     """One pmp_ctx: device memory, a HIP stream and an image pool."""

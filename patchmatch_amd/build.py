@@ -19,7 +19,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_objs")
 LIB = os.path.join(HERE, "libpmp.so")
-SOURCES = ["pmp_context.hip", "pmp_estimator.hip", "pmp_queue.hip", "pmp_preinit.hip", "pmp_server.hip", "pmp_sched.cpp", "pmp_fusion.hip", "pmp_io.hip", "pmp_thumb.hip", "pmp_frontend.hip", "pmp_debug.hip"]
+SOURCES = ["pmp_context.hip", "pmp_estimator.hip", "pmp_queue.hip", "pmp_preinit.hip", "pmp_server.hip", "pmp_sched.cpp", "pmp_trace.cpp", "pmp_fusion.hip", "pmp_io.hip", "pmp_thumb.hip", "pmp_frontend.hip", "pmp_debug.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("PMP_OFFLOAD_ARCH", "gfx950")

@@ -5,6 +5,7 @@
 #include <string>
 #include <vector>
 
+#include "pmp_trace.h"
 #include "pmp_internal.h"
 
 namespace pmp {
@@ -12,6 +13,28 @@ namespace pmp {
 int set_error(pmp_ctx* ctx, int code, const std::string& msg) {
     if (ctx) ctx->last_error = msg;
     return code;
+}
+
+int upload_host_rows(pmp_ctx* ctx, void* dst, const uint8_t* src, size_t row, size_t rows, size_t pitch,
+                     hipStream_t s) {
+    const size_t bytes = row * rows;
+    if (bytes == 0) return PMP_OK;
+    if (bytes > ctx->stage_cap) {
+        if (ctx->stage) (void)hipHostFree(ctx->stage);
+        ctx->stage = nullptr;
+        ctx->stage_cap = 0;
+        PMP_HIP(ctx, hipHostMalloc((void**)&ctx->stage, bytes, hipHostMallocDefault));
+        ctx->stage_cap = bytes;
+    }
+    if (pitch == row) {
+        memcpy(ctx->stage, src, bytes);
+    } else {
+        for (size_t r = 0; r < rows; ++r) memcpy(ctx->stage + r * row, src + r * pitch, row);
+    }
+    PMP_HIP(ctx, hipMemcpyAsync(dst, ctx->stage, bytes, hipMemcpyHostToDevice, s));
+    // the staging buffer is reused by the next upload
+    PMP_HIP(ctx, hipStreamSynchronize(s));
+    return PMP_OK;
 }
 
 int check_hip(pmp_ctx* ctx, hipError_t e, const char* what) {
@@ -256,6 +279,7 @@ void pmp_destroy(pmp_ctx* ctx) {
                     ctx->epochs};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
+    if (ctx->stage) (void)hipHostFree(ctx->stage);
     for (auto& kv : ctx->rowtabs) (void)hipFree(kv.second);
     for (auto& kv : ctx->coltabs) (void)hipFree(kv.second);
     for (auto& kv : ctx->wcs) (void)hipFree(kv.second);
@@ -338,6 +362,7 @@ int pmp_get_timing(pmp_ctx* ctx, pmp_timing* out) {
 
 int pmp_image_put(pmp_ctx* ctx, uint64_t id, const uint8_t* pixels, int32_t width, int32_t height, int64_t pitch,
                   int32_t on_device) {
+    TraceRange trace_range("pmp_image_put");
     if (!ctx || !pixels || width <= 0 || height <= 0 || pitch < width) return PMP_EINVAL;
     PMP_HIP(ctx, hipSetDevice(ctx->device));
     auto it = ctx->images.find(id);
@@ -352,12 +377,18 @@ int pmp_image_put(pmp_ctx* ctx, uint64_t id, const uint8_t* pixels, int32_t widt
     const size_t bytes = (size_t)width * height;
     PMP_HIP(ctx, hipMalloc(&im.gray, bytes));
     im.bytes = bytes;
-    hipError_t e = hipMemcpy2DAsync(im.gray, width, pixels, (size_t)pitch, width, height,
-                                    on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, ctx->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-    if (e != hipSuccess) {
+    int rc;
+    if (on_device) {
+        hipError_t e = hipMemcpy2DAsync(im.gray, width, pixels, (size_t)pitch, width, height, hipMemcpyDeviceToDevice,
+                                        ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        rc = e == hipSuccess ? PMP_OK : check_hip(ctx, e, "pmp_image_put copy");
+    } else {
+        rc = upload_host_rows(ctx, im.gray, pixels, (size_t)width, (size_t)height, (size_t)pitch, ctx->stream);
+    }
+    if (rc) {
         free_image(im);
-        return check_hip(ctx, e, "pmp_image_put copy");
+        return rc;
     }
     ctx->images[id] = im;
     return PMP_OK;

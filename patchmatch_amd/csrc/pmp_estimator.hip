@@ -17,6 +17,7 @@
 #include <cstdio>
 #include <string>
 
+#include "pmp_trace.h"
 #include "pmp_internal.h"
 #include "pmp_math.h"
 #include "pmp_queue.h"
@@ -226,6 +227,7 @@ int pmp_set_mask(pmp_ctx* ctx, int64_t mask_id) {
 }
 
 int pmp_random_initialization(pmp_ctx* ctx, const pmp_params* params) {
+    TraceRange trace_range("pmp_random_initialization");
     if (!ctx || !params) return PMP_EINVAL;
     QueueState* q = nullptr;
     QueueRun r;
@@ -238,6 +240,7 @@ int pmp_random_initialization(pmp_ctx* ctx, const pmp_params* params) {
 }
 
 int pmp_score_plane_depth(pmp_ctx* ctx, const pmp_params* params) {
+    TraceRange trace_range("pmp_score_plane_depth");
     if (!ctx || !params) return PMP_EINVAL;
     QueueState* q = nullptr;
     QueueRun r;
@@ -247,6 +250,7 @@ int pmp_score_plane_depth(pmp_ctx* ctx, const pmp_params* params) {
 }
 
 int pmp_red_black_pass(pmp_ctx* ctx, const pmp_params* params, int32_t it_begin, int32_t it_end) {
+    TraceRange trace_range("pmp_red_black_pass");
     if (!ctx || !params || it_begin < 0 || it_end < it_begin) return PMP_EINVAL;
     if (it_end - it_begin > kMaxPasses / 2) return set_error(ctx, PMP_EINVAL, "at most 64 iterations per call");
     QueueState* q = nullptr;
@@ -265,6 +269,7 @@ int pmp_red_black_pass(pmp_ctx* ctx, const pmp_params* params, int32_t it_begin,
 }
 
 int pmp_finalize(pmp_ctx* ctx, const pmp_params* params, const uint8_t* mask) {
+    TraceRange trace_range("pmp_finalize");
     if (!ctx || !params) return PMP_EINVAL;
     if (!ctx->views_set) return set_error(ctx, PMP_ESTATE, "pmp_set_views has not been called");
     const uint8_t* dmask = nullptr;
@@ -280,6 +285,7 @@ int pmp_finalize(pmp_ctx* ctx, const pmp_params* params, const uint8_t* mask) {
 }
 
 int pmp_densify(pmp_ctx* ctx, const pmp_params* params, const uint8_t* mask) {
+    TraceRange trace_range("pmp_densify");
     if (!ctx || !params || params->iterations < 0) return PMP_EINVAL;
     if (!ctx->views_set) return set_error(ctx, PMP_ESTATE, "pmp_set_views has not been called");
     const uint8_t* dmask = nullptr;

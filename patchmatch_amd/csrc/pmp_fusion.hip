@@ -26,6 +26,7 @@
 #include <mutex>
 #include <vector>
 
+#include "pmp_trace.h"
 #include "pmp_internal.h"
 #include "pmp_geo.h"
 #include "pmp_math.h"
@@ -389,6 +390,7 @@ static int clean_impl(pmp_ctx* ctx, int32_t width, int32_t height, int32_t n_map
 
 int pmp_clean(pmp_ctx* ctx, int32_t width, int32_t height, int32_t n_maps, const float* const* depth_maps,
               const pmp_pose* poses, float same_depth_thr, int32_t min_consistent, float* cleaned) {
+    TraceRange trace_range("pmp_clean");
     return clean_impl(ctx, width, height, n_maps, depth_maps, nullptr, poses, same_depth_thr, min_consistent, 0.0f,
                       0.0f, cleaned);
 }
@@ -396,6 +398,7 @@ int pmp_clean(pmp_ctx* ctx, int32_t width, int32_t height, int32_t n_maps, const
 int pmp_clean_ex(pmp_ctx* ctx, int32_t width, int32_t height, int32_t n_maps, const float* const* depth_maps,
                  const float* const* plane4_maps, const pmp_pose* poses, float same_depth_thr, int32_t min_consistent,
                  float max_reproj_px, float max_normal_deg, float* cleaned) {
+    TraceRange trace_range("pmp_clean_ex");
     return clean_impl(ctx, width, height, n_maps, depth_maps, plane4_maps, poses, same_depth_thr, min_consistent,
                       max_reproj_px, max_normal_deg, cleaned);
 }
@@ -481,6 +484,7 @@ int pmp_pruner_add_view(pmp_ctx* ctx, const float* cleaned_depth, const pmp_pose
 
 int pmp_pruner_prune(pmp_ctx* ctx, float same_depth_thr, float* pruned_depth, int64_t cap, double* pts, uint8_t* cols,
                      int32_t* kps, int64_t* count) {
+    TraceRange trace_range("pmp_pruner_prune");
     if (!ctx || cap < 0 || (cap > 0 && (!pts || !cols || !kps))) return PMP_EINVAL;
     if (ctx->pruner.empty()) {
         if (count) *count = 0;

@@ -166,6 +166,10 @@ struct pmp_ctx {
     size_t epochs_cap = 0;
     std::vector<int> epoch_tiles;
     size_t epoch_words = 0;
+
+    // pinned staging of host uploads (upload_host_rows)
+    uint8_t* stage = nullptr;
+    size_t stage_cap = 0;
 };
 
 namespace pmp {
@@ -176,6 +180,12 @@ enum { kStageInit = 0, kStagePropagate = 1, kStageFinalize = 2 };
 int timing_begin(pmp_ctx* ctx, int stage);
 void timing_end(pmp_ctx* ctx, int pair);
 int check_hip(pmp_ctx* ctx, hipError_t e, const char* what);
+// `rows` host rows of `row` bytes, `pitch` bytes apart, to contiguous device
+// memory at dst on stream s, and waits for the copy.  The rows are packed into
+// a pinned staging buffer of the context (grown on demand) and sent in one
+// copy: a 2D copy from pageable memory moves row by row, about 9 us a row.
+int upload_host_rows(pmp_ctx* ctx, void* dst, const uint8_t* src, size_t row, size_t rows, size_t pitch,
+                     hipStream_t s);
 int ensure_workspace(pmp_ctx* ctx, int W, int H);
 struct Tables {
     const float2* row = nullptr;

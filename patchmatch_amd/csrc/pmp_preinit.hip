@@ -21,6 +21,7 @@
 #include <string.h>
 
 #include "pmp_geo.h"
+#include "pmp_trace.h"
 #include "pmp_internal.h"
 #include "pmp_math.h"
 #include "pmp_queue.h"
@@ -332,6 +333,7 @@ extern "C" {
 
 int pmp_preinit_landmarks(pmp_ctx* ctx, const pmp_params* params, int64_t n, const double* xyz, const uint8_t* mask,
                           int64_t* accepted) {
+    TraceRange trace_range("pmp_preinit_landmarks");
     if (!ctx || !params || n < 0 || (n > 0 && !xyz)) return PMP_EINVAL;
     if (n > 0x7FFFFFFF) return set_error(ctx, PMP_EINVAL, "too many landmarks");
     if (accepted) *accepted = 0;
@@ -384,6 +386,7 @@ int pmp_forget_result(pmp_ctx* ctx) {
 }
 
 int pmp_preinit_warp(pmp_ctx* ctx, const pmp_params* params, const uint8_t* mask, int64_t* accepted) {
+    TraceRange trace_range("pmp_preinit_warp");
     if (!ctx || !params) return PMP_EINVAL;
     if (accepted) *accepted = 0;
     if (!ctx->views_set) return set_error(ctx, PMP_ESTATE, "pmp_set_views has not been called");

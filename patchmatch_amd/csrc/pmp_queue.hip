@@ -18,6 +18,7 @@
 #include <math.h>
 #include <string.h>
 
+#include "pmp_trace.h"
 #include "pmp_internal.h"
 #include "pmp_math.h"
 #include "pmp_queue.h"
@@ -551,6 +552,7 @@ using namespace pmp;
 extern "C" {
 
 int pmp_batch_densify(pmp_ctx* ctx, const pmp_params* params, int32_t n, const pmp_keyframe* kfs) {
+    TraceRange trace_range("pmp_batch_densify");
     if (!ctx || !params || !kfs) return PMP_EINVAL;
     if (n < 1 || n > kMaxSlots) return set_error(ctx, PMP_EINVAL, "batch size must be in [1, PMP_MAX_BATCH]");
     PMP_HIP(ctx, hipSetDevice(ctx->device));
@@ -602,6 +604,7 @@ int pmp_batch_get_config(pmp_ctx* ctx, pmp_batch_config* cfg) {
 
 int pmp_batch_autotune(pmp_ctx* ctx, const pmp_params* params, int32_t n, const pmp_keyframe* kfs,
                        pmp_batch_config* best) {
+    TraceRange trace_range("pmp_batch_autotune");
     if (!ctx || !params || !kfs) return PMP_EINVAL;
     if (n < 1 || n > kMaxSlots) return set_error(ctx, PMP_EINVAL, "batch size must be in [1, PMP_MAX_BATCH]");
     PMP_HIP(ctx, hipSetDevice(ctx->device));

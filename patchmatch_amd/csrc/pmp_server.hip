@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "pmp_internal.h"
+#include "pmp_trace.h"
 #include "pmp_queue.h"
 #include "pmp_sched.h"
 
@@ -89,6 +90,7 @@ class GpuExec final : public sched::Executor {
     int64_t map_bytes(int64_t pixels) const override { return pixels * (int64_t)SlotBuf::kBytesPerPixel; }
 
     int upload(uint64_t id, const uint8_t* pixels, int W, int H, std::string& err) override {
+        TraceRange trace_range("pmp_server upload");
         (void)hipSetDevice(ctx->device);
         Image im;
         im.W = W;
@@ -150,6 +152,7 @@ class GpuExec final : public sched::Executor {
 
     int step_issue(int w, const std::vector<sched::StepItem>& items, std::vector<int>& item_rc,
                    std::vector<std::string>& item_err, std::string& err) override {
+        TraceRange trace_range("pmp_server step");
         DescSet& d = desc[w];
         int items_init[kMaxSlots] = {0}, items_prop[kMaxSlots] = {0}, items_rows[kMaxSlots] = {0};
         bool post = false;

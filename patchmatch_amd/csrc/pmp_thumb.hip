@@ -96,8 +96,7 @@ static int stage_src(pmp_ctx* ctx, const uint8_t* pixels, int W, int H, int64_t 
     const size_t row = (size_t)W * C;
     int rc = grow(ctx, ctx->thumb.src, ctx->thumb.src_cap, row * H);
     if (rc) return rc;
-    PMP_HIP(ctx, hipMemcpy2DAsync(ctx->thumb.src, row, pixels, (size_t)pitch, row, H, hipMemcpyHostToDevice,
-                                  ctx->stream));
+    if ((rc = upload_host_rows(ctx, ctx->thumb.src, pixels, row, (size_t)H, (size_t)pitch, ctx->stream))) return rc;
     dsrc = ctx->thumb.src;
     dpitch = (int64_t)row;
     return PMP_OK;

@@ -89,3 +89,11 @@ def test_batch_and_server_struct_layout(tmp_path):
     subprocess.run(["gcc", "-std=c11", f"-I{os.path.join(ROOT, 'include')}", str(src), "-o", str(exe)], check=True)
     got = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
     assert got == [w for _, w in fields], list(zip([e for e, _ in fields], got, [w for _, w in fields]))
+
+
+def test_roctx_toggle():
+    """pmp_set_roctx needs no device: the ROCm marker library of this image
+    opens, and tracing turns off again."""
+    from patchmatch_amd import _lib
+    assert _lib.set_roctx(True)
+    assert _lib.set_roctx(False)
