@@ -334,7 +334,7 @@ def main(argv=None):
             st = workloads.Stream(args.seed + 3, workloads.SERVE_CLASSES, per_class=125, with_gt=False)
             print(f"[bench] served workload rendered in {time.perf_counter() - t:.1f}s", file=sys.stderr, flush=True)
             rates = [float(r) for r in args.rates.split(",") if r]
-            result["served"] = workloads.run_served(st, p, rates, args.served_requests, args.seed + 4)
+            result["served"] = workloads.run_served(st, p, rates, args.served_requests, args.seed + 4, tail_stages=2)
             print(f"[bench] served: {result['served']}", file=sys.stderr, flush=True)
             # every request of a stream against plain, under a budget that holds the maps of two
             # 5760x2880 slots and the images of two requests. Requests 0 and 1 are 5760x2880 so that
@@ -355,7 +355,7 @@ def main(argv=None):
             # the plain workload's shape alone (1920x960, 6 neighbours) below capacity: the latency target's case
             st = workloads.Stream(args.seed + 7, [(1920, 960, 1.0)], per_class=16, with_gt=False)
             result["served_1920x960"] = workloads.run_served(st, p, [], args.served_requests, args.seed + 8,
-                                                             views=(6, 6))
+                                                             views=(6, 6), tail_stages=2)
             print(f"[bench] served 1920x960: {result['served_1920x960']}", file=sys.stderr, flush=True)
             del st
     print(json.dumps(result))

@@ -319,6 +319,12 @@ typedef struct pmp_server_opts {
                                    it stay within the budget, or when nothing is in flight; 0: no
                                    budget (only max_slots limits) */
     pmp_batch_config config;    /* tile_rows, blocks_per_cu, xcd_bands of the step kernels */
+    int32_t tail_stages;        /* latency-first admission: while at most one request is queued, it
+                                   joins the requests in flight only once each of them has at most
+                                   this many stages left (a request's stages: init, 2 per
+                                   iteration, finalize), so that they are not slowed by sharing
+                                   their pass launches; with two or more queued, admission is free
+                                   (throughput first); 0: always free */
 } pmp_server_opts;
 
 /* One densification request: the keyframe (pooled ids, poses, mask id, depth

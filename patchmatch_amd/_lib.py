@@ -95,7 +95,8 @@ class Timing(ctypes.Structure):
 
 class ServerOpts(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int32), ("max_slots", ctypes.c_int32), ("hbm_budget", ctypes.c_int64),
-                ("edge_mode", ctypes.c_int32), ("inflight_kpx", ctypes.c_int32), ("config", BatchConfig)]
+                ("edge_mode", ctypes.c_int32), ("inflight_kpx", ctypes.c_int32), ("config", BatchConfig),
+                ("tail_stages", ctypes.c_int32)]
 
 
 class Request(ctypes.Structure):
@@ -605,10 +606,10 @@ class Server:
     """pmp_server: densification requests served with continuous batching."""
 
     def __init__(self, device=0, max_slots=16, hbm_budget=0, edge_mode=0, tile_rows=8, blocks_per_cu=0,
-                 xcd_bands=1, inflight_kpx=0):
+                 xcd_bands=1, inflight_kpx=0, tail_stages=0):
         self.L = load()
         o = ServerOpts(device, max_slots, hbm_budget, edge_mode, inflight_kpx,
-                       BatchConfig(tile_rows, blocks_per_cu, 1, xcd_bands))
+                       BatchConfig(tile_rows, blocks_per_cu, 1, xcd_bands), tail_stages)
         h = _vp()
         rc = self.L.pmp_server_create(ctypes.byref(h), ctypes.byref(o))
         if rc != 0:

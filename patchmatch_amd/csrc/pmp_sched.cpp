@@ -163,6 +163,16 @@ void Server::admit() {
         }
         if (slot < 0) return;
         if (o_.inflight_kpx > 0 && inflight > 0 && inflight + P > 1000ll * o_.inflight_kpx) return;
+        // tail admission: without a backlog (one request queued), a request
+        // joins the ones in flight only for their last tail_stages stages, so
+        // that they finish about when they would alone (sharing every pass
+        // launch would delay the oldest); with a backlog, admission is free
+        if (o_.tail_stages > 0 && inflight > 0 && queue_.size() < 2) {
+            for (int k = 0; k < o_.max_slots; ++k) {
+                const Req* q = slot_req_[k];
+                if (q && !q->ending && q->job.last_stage + 1 - q->stage > o_.tail_stages) return;
+            }
+        }
         if (P > slot_cap_[slot]) {
             const int64_t grow = ex_->map_bytes(P) - ex_->map_bytes(slot_cap_[slot]);
             if (!make_room(grow)) {

@@ -96,6 +96,7 @@ struct Options {
     int64_t budget = 64ll << 30;
     int edge_mode = PMP_EDGE_WRAP;
     int inflight_kpx = 0;
+    int tail_stages = 0;
 };
 
 class Server {

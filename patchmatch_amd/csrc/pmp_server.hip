@@ -268,7 +268,7 @@ int pmp_server_create(pmp_server** out, const pmp_server_opts* opts) {
     if (o.hbm_budget == 0) o.hbm_budget = 64ll << 30;
     if (o.config.tile_rows == 0) o.config = pmp_batch_config{8, 0, 1, 1};
     if (o.max_slots < 1 || o.max_slots > kMaxSlots || o.hbm_budget < 0 || o.edge_mode < 0 || o.edge_mode > 2 ||
-        o.inflight_kpx < 0 ||
+        o.inflight_kpx < 0 || o.tail_stages < 0 ||
         (o.config.tile_rows != 2 && o.config.tile_rows != 4 && o.config.tile_rows != 8))
         return fail(PMP_EINVAL, "invalid server options");
     pmp_server* s = new (std::nothrow) pmp_server();
@@ -283,6 +283,7 @@ int pmp_server_create(pmp_server** out, const pmp_server_opts* opts) {
     so.budget = o.hbm_budget;
     so.edge_mode = o.edge_mode;
     so.inflight_kpx = o.inflight_kpx;
+    so.tail_stages = o.tail_stages;
     s->srv = new (std::nothrow) sched::Server(&s->exec, so);
     if (!s->srv) {
         delete s;

@@ -321,10 +321,12 @@ def run_plain_stream(stream: Stream, params, specs, arrivals):
     return _summary(lat, arrivals[0], max(done), len(specs), _labels(stream, specs))
 
 
-def run_served_stream(stream: Stream, params, specs, arrivals, max_slots=16, hbm_budget=64 << 30, inflight_kpx=0):
+def run_served_stream(stream: Stream, params, specs, arrivals, max_slots=16, hbm_budget=64 << 30, inflight_kpx=0,
+                      tail_stages=0):
     """Requests submitted at their arrival times to a pmp_server."""
     from ._lib import Server
-    srv = Server(0, max_slots=max_slots, hbm_budget=hbm_budget, edge_mode=params.edge_mode, inflight_kpx=inflight_kpx)
+    srv = Server(0, max_slots=max_slots, hbm_budget=hbm_budget, edge_mode=params.edge_mode, inflight_kpx=inflight_kpx,
+                 tail_stages=tail_stages)
     # warm the kernels once (not timed)
     kf = stream.keyframe(specs[0])
     t = srv.submit(params, kf["ref_id"], kf["ref_pose"], kf["view_ids"], kf["view_poses"], kf["shape"],
@@ -373,12 +375,14 @@ def run_served_stream(stream: Stream, params, specs, arrivals, max_slots=16, hbm
     return out
 
 
-def run_served(stream: Stream, params, rates, n_requests: int, seed: int, views=(2, 10), below=0.5):
+def run_served(stream: Stream, params, rates, n_requests: int, seed: int, views=(2, 10), below=0.5, tail_stages=0):
     """The same request list at every rate; only the arrival times change.
     First every request arrives at once: the completion rate of that
     saturated run is each form's capacity.  Then the given rates, and
     `below` x the served capacity, where arrivals do not outrun the device
-    and the latencies are those of the requests rather than of a backlog."""
+    and the latencies are those of the requests rather than of a backlog.
+    With tail_stages > 0, the below-capacity rate is also served with that
+    latency-first admission (pmp_server_opts.tail_stages)."""
     out = {}
     specs = stream.draw(np.random.default_rng(seed), n_requests, views)
     at_once = np.zeros(n_requests)
@@ -397,6 +401,12 @@ def run_served(stream: Stream, params, rates, n_requests: int, seed: int, views=
         out[f"{rate:g}_per_s"] = {"plain": plain, "served": served,
                                   "throughput_ratio": round(served["throughput_per_s"] / plain["throughput_per_s"], 3),
                                   "p99_ratio": round(served["p99_ms"] / plain["p99_ms"], 3)}
+        if tail_stages and below and rate == rates[-1]:
+            lf = run_served_stream(stream, params, specs, arr, tail_stages=tail_stages)
+            print(f"[workloads] {rate:g} req/s served, tail_stages {tail_stages}: {lf}", file=sys.stderr, flush=True)
+            out[f"{rate:g}_per_s"]["served_latency_first"] = dict(
+                lf, tail_stages=tail_stages, p50_ratio=round(lf["p50_ms"] / plain["p50_ms"], 3),
+                p99_ratio=round(lf["p99_ms"] / plain["p99_ms"], 3))
     return out
 
 

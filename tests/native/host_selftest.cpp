@@ -59,6 +59,7 @@ struct SimExec final : sched::Executor {
     std::vector<Write> pending[2];
     std::set<uint64_t> reading[2];           // images read by the step in flight in each set
     std::map<uint64_t, std::vector<int>> stages;   // ticket -> stages run, in order
+    std::map<std::pair<uint64_t, int>, int> step_of;   // (ticket, stage) -> index of the step that issued it
     std::atomic<int> issues{0}, waits{0};
     // fault injection
     uint64_t fail_upload_id = ~0ull, fail_pack_id = ~0ull;
@@ -170,6 +171,7 @@ struct SimExec final : sched::Executor {
             auto& st = stages[j->ticket];
             if ((int)st.size() != it.stage) bad("stage out of order");
             st.push_back(it.stage);
+            step_of[{j->ticket, it.stage}] = n;
             if (it.stage == 0 && j->n_lm > 0) {
                 volatile double sink = j->lm[3 * j->n_lm - 1];   // the landmarks must be readable
                 (void)sink;
@@ -598,6 +600,43 @@ static void test_slot_best_fit() {
     CHECK(ex.violations == 0);
 }
 
+// 7. Tail admission: with one request queued, it joins a request in flight
+// only for that request's last tail_stages stages; with a backlog of two or
+// more, admission is free.  (3 iterations: stages 0..7.)
+static void test_tail_admission() {
+    const int W = 16, H = 8;
+    for (int mode = 0; mode < 3; ++mode) {   // 0: tail 2; 1: tail off; 2: tail 2 with a backlog
+        SimExec ex(1ll << 30, 2);
+        ex.step_us = 1000;   // the second submit comes microseconds after the first
+        sched::Options o;
+        o.max_slots = 2;
+        o.budget = 1ll << 30;
+        o.tail_stages = mode == 1 ? 0 : 2;
+        Images im;
+        uint64_t a, b, c = 0;
+        {
+            sched::Server s(&ex, o);
+            std::string err;
+            if (mode == 2) s.pause(true);
+            CHECK(s.submit(request(im, 1, 10, 2, W, H), params(3, 1), &a, err) == PMP_OK);
+            CHECK(s.submit(request(im, 2, 20, 2, W, H), params(3, 2), &b, err) == PMP_OK);
+            if (mode == 2) {
+                CHECK(s.submit(request(im, 3, 30, 2, W, H), params(3, 3), &c, err) == PMP_OK);
+                s.pause(false);
+            }
+            CHECK(fetch(s, a, W, H) == PMP_OK);
+            CHECK(fetch(s, b, W, H) == PMP_OK);
+            if (mode == 2) CHECK(fetch(s, c, W, H) == PMP_OK);
+        }
+        std::lock_guard<std::mutex> lk(ex.mu);
+        const int b0 = ex.step_of.at({b, 0});
+        if (mode == 0) CHECK(b0 == ex.step_of.at({a, 6}));   // joins for a's last 2 stages
+        if (mode == 1) CHECK(b0 <= ex.step_of.at({a, 2}));   // joins at once
+        if (mode == 2) CHECK(b0 == ex.step_of.at({a, 0}));   // backlog: joins at once
+        CHECK(ex.violations == 0);
+    }
+}
+
 // ---------------------------------------------------------------- host-only library code
 static void test_frontend_and_writers() {
     // depth interval: 1e-5 and 1.1 x the distance at sorted index n/10*9
@@ -673,6 +712,7 @@ int main(int argc, char** argv) {
     test_faults();
     test_destroy_finishes();
     test_slot_best_fit();
+    test_tail_admission();
     if (!sched_only) test_frontend_and_writers();
     printf("host_selftest: ok\n");
     return 0;

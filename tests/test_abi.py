@@ -74,6 +74,7 @@ def test_batch_and_server_struct_layout(tmp_path):
               ("sizeof(pmp_batch_config)", ctypes.sizeof(_lib.BatchConfig)),
               ("sizeof(pmp_server_opts)", ctypes.sizeof(_lib.ServerOpts)),
               ("offsetof(pmp_server_opts, config)", _lib.ServerOpts.config.offset),
+              ("offsetof(pmp_server_opts, tail_stages)", _lib.ServerOpts.tail_stages.offset),
               ("sizeof(pmp_request)", ctypes.sizeof(_lib.Request)),
               ("offsetof(pmp_request, ref_pixels)", _lib.Request.ref_pixels.offset),
               ("offsetof(pmp_request, mask_pixels)", _lib.Request.mask_pixels.offset),

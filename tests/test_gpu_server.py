@@ -92,6 +92,25 @@ def test_server_same_alone_and_in_the_mix(gpu_ctx, slots):
         assert st["max_in_flight"] == min(slots, len(kfs))
 
 
+def test_server_latency_first_admission(gpu_ctx):
+    """tail_stages = 2: requests submitted one after another (each while the
+    previous is in flight) join it only for its last 2 stages, unless a backlog
+    builds up; every result is unchanged."""
+    kfs = _keyframes()
+    want = _plain(gpu_ctx, kfs)
+    print("[server] latency-first admission, requests one after another", flush=True)
+    with Server(0, max_slots=4, tail_stages=2) as srv:
+        tickets = []
+        for k in range(len(kfs)):
+            tickets.append(_submit(srv, kfs, k))
+            time.sleep(0.002)
+        for k, t in enumerate(tickets):
+            assert srv.wait(t, 60000), f"request {k} not done in 60 s"
+            _check(srv.fetch(t), want[k], f"latency-first request {k}")
+        st = srv.stats()
+        assert st["completed"] == len(kfs) and st["failed"] == 0
+
+
 def test_server_inflight_pixel_budget(gpu_ctx):
     """An admission budget of 120 thousand pixels: requests are admitted oldest
     first while the keyframes in flight stay within it (a larger one alone),
