@@ -37,6 +37,11 @@ class GpuExec final : public sched::Executor {
     // ctx->images and ctx->last_error: touched by the submitting thread
     // (upload, pack) and the scheduler thread (drop, slot_grow, step_issue)
     std::mutex pool_mu;
+    // pinned staging of uploads (one copy from pinned memory instead of a
+    // pageable one), shared by the submitting threads
+    std::mutex stage_mu;
+    uint8_t* stage = nullptr;
+    size_t stage_cap = 0;
 
     int init(const pmp_server_opts& o) {
         cfg = o.config;
@@ -68,6 +73,7 @@ class GpuExec final : public sched::Executor {
         for (hipEvent_t e : step_done)
             if (e) (void)hipEventDestroy(e);
         if (copy) (void)hipStreamDestroy(copy);
+        if (stage) (void)hipHostFree(stage);
         pmp_destroy(ctx);   // frees the pooled images
     }
 
@@ -97,8 +103,22 @@ class GpuExec final : public sched::Executor {
         im.H = H;
         im.bytes = (size_t)W * H;
         hipError_t e = hipMalloc(&im.gray, im.bytes);
-        if (e == hipSuccess) e = hipMemcpyAsync(im.gray, pixels, im.bytes, hipMemcpyHostToDevice, copy);
-        if (e == hipSuccess) e = hipStreamSynchronize(copy);
+        if (e == hipSuccess) {
+            std::lock_guard<std::mutex> lk(stage_mu);
+            if (im.bytes > stage_cap) {
+                if (stage) (void)hipHostFree(stage);
+                stage = nullptr;
+                stage_cap = 0;
+                e = hipHostMalloc((void**)&stage, im.bytes, hipHostMallocDefault);
+                if (e == hipSuccess) stage_cap = im.bytes;
+            }
+            if (e == hipSuccess) {
+                memcpy(stage, pixels, im.bytes);
+                e = hipMemcpyAsync(im.gray, stage, im.bytes, hipMemcpyHostToDevice, copy);
+            }
+            // the staging buffer is reused by the next upload
+            if (e == hipSuccess) e = hipStreamSynchronize(copy);
+        }
         if (e != hipSuccess) {
             if (im.gray) (void)hipFree(im.gray);
             err = "upload of image " + std::to_string(id) + ": " + hipGetErrorString(e);
