@@ -333,7 +333,7 @@ static int clean_impl(pmp_ctx* ctx, int32_t width, int32_t height, int32_t n_map
                       int32_t min_consistent, float max_reproj_px, float max_normal_deg, float* cleaned) {
     if (!ctx || !depth_maps || !poses || !cleaned || width < 2 || height < 2) return PMP_EINVAL;
     if (n_maps < 1 || n_maps > kMaxMaps) return set_error(ctx, PMP_EINVAL, "n_maps must be in [1, 16]");
-    if (!(max_reproj_px <= 0.0f || max_reproj_px > 0.0f) || !(max_normal_deg < 90.0f))
+    if (isnan(max_reproj_px) || !(max_normal_deg < 90.0f))
         return set_error(ctx, PMP_EINVAL, "max_reproj_px must not be NaN; max_normal_deg must be below 90");
     const bool use_normals = max_normal_deg > 0.0f;
     if (use_normals && !plane4_maps) return set_error(ctx, PMP_EINVAL, "the normal test needs the plane maps");
