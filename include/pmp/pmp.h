@@ -118,7 +118,9 @@ int pmp_synchronize(pmp_ctx* ctx);
  * Replaces InitTextureImages (PatchMatchCuda.cu:697-797, float textures built
  * from depthmap.cc:569-571); images stay resident as uint8 plus a packed
  * 2x2-neighbourhood copy for bilinear sampling.  on_device != 0: pixels is a
- * device pointer on the context's device. */
+ * device pointer on the context's device.  Host rows (pitch bytes apart) go
+ * through a pinned staging buffer of the context in one copy; the call
+ * returns once the image is on the device. */
 int pmp_image_put(pmp_ctx* ctx, uint64_t id, const uint8_t* pixels, int32_t width, int32_t height,
                   int64_t pitch, int32_t on_device);
 int pmp_image_drop(pmp_ctx* ctx, uint64_t id);
