@@ -102,7 +102,9 @@ typedef struct pmp_pose {
 
 /* Replaces the per-call cudaMalloc/cudaFree of DepthmapEstimator
  * (depthmap.cc:522,557-558,652-656,771-800): buffers live in the context and
- * grow on demand.  device: HIP device ordinal. */
+ * grow on demand.  device: HIP device ordinal.  Every call on a context (and
+ * on a server) runs on that device from any thread and leaves the calling
+ * thread's current HIP device as it found it. */
 int pmp_create(pmp_ctx** out, int32_t device);
 void pmp_destroy(pmp_ctx* ctx);
 /* Replaces printing cudaGetErrorString and continuing (PatchMatchCuda.cu:447-451). */

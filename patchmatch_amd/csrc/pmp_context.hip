@@ -257,7 +257,8 @@ int pmp_create(pmp_ctx** out, int32_t device) {
     pmp_ctx* ctx = new (std::nothrow) pmp_ctx();
     if (!ctx) return PMP_ENOMEM;
     ctx->device = device;
-    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+    DeviceScope dev(device);
+    if (!dev.ok() || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
         delete ctx;
         return PMP_EDEVICE;
     }
@@ -267,7 +268,7 @@ int pmp_create(pmp_ctx** out, int32_t device) {
 
 void pmp_destroy(pmp_ctx* ctx) {
     if (!ctx) return;
-    (void)hipSetDevice(ctx->device);
+    DeviceScope dev(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     queue_free(ctx);
     preinit_free(ctx);
@@ -329,6 +330,7 @@ void pmp_default_params(pmp_params* p) {
 
 int pmp_synchronize(pmp_ctx* ctx) {
     if (!ctx) return PMP_EINVAL;
+    PMP_ON_DEVICE(ctx);
     PMP_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return PMP_OK;
 }
@@ -343,6 +345,7 @@ int pmp_set_timing(pmp_ctx* ctx, int32_t enable) {
 
 int pmp_get_timing(pmp_ctx* ctx, pmp_timing* out) {
     if (!ctx || !out) return PMP_EINVAL;
+    PMP_ON_DEVICE(ctx);
     memset(out, 0, sizeof(*out));
     PMP_HIP(ctx, hipStreamSynchronize(ctx->stream));
     for (size_t k = 0; k < ctx->events_used; ++k) {
@@ -364,7 +367,7 @@ int pmp_image_put(pmp_ctx* ctx, uint64_t id, const uint8_t* pixels, int32_t widt
                   int32_t on_device) {
     TraceRange trace_range("pmp_image_put");
     if (!ctx || !pixels || width <= 0 || height <= 0 || pitch < width) return PMP_EINVAL;
-    PMP_HIP(ctx, hipSetDevice(ctx->device));
+    PMP_ON_DEVICE(ctx);
     auto it = ctx->images.find(id);
     if (it != ctx->images.end()) {
         PMP_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -396,6 +399,7 @@ int pmp_image_put(pmp_ctx* ctx, uint64_t id, const uint8_t* pixels, int32_t widt
 
 int pmp_image_get(pmp_ctx* ctx, uint64_t id, uint8_t* pixels) {
     if (!ctx || !pixels) return PMP_EINVAL;
+    PMP_ON_DEVICE(ctx);
     auto it = ctx->images.find(id);
     if (it == ctx->images.end()) return set_error(ctx, PMP_ENOTFOUND, "unknown image id");
     PMP_HIP(ctx, hipMemcpyAsync(pixels, it->second.gray, it->second.bytes, hipMemcpyDeviceToHost, ctx->stream));
@@ -405,6 +409,7 @@ int pmp_image_get(pmp_ctx* ctx, uint64_t id, uint8_t* pixels) {
 
 int pmp_image_drop(pmp_ctx* ctx, uint64_t id) {
     if (!ctx) return PMP_EINVAL;
+    PMP_ON_DEVICE(ctx);
     auto it = ctx->images.find(id);
     if (it == ctx->images.end()) return set_error(ctx, PMP_ENOTFOUND, "unknown image id");
     PMP_HIP(ctx, hipStreamSynchronize(ctx->stream));

@@ -64,6 +64,7 @@ extern "C" {
 
 int pmp_debug_eval_math(pmp_ctx* ctx, int32_t which, int64_t n, const float* a, const float* b, float* out) {
     if (!ctx || !a || !out || n < 0 || n > (1ll << 28) || which < 0 || which > 11) return PMP_EINVAL;
+    PMP_ON_DEVICE(ctx);
     if (n == 0) return PMP_OK;
     int rc = ensure_dbg(ctx, 3 * (size_t)n);
     if (rc) return rc;
@@ -82,6 +83,7 @@ int pmp_debug_eval_math(pmp_ctx* ctx, int32_t which, int64_t n, const float* a, 
 
 int pmp_debug_copy(pmp_ctx* ctx, int64_t bytes, int32_t repeats) {
     if (!ctx || bytes <= 0 || bytes % 16 != 0 || bytes > (1ll << 33) || repeats < 1) return PMP_EINVAL;
+    PMP_ON_DEVICE(ctx);
     int rc = ensure_dbg(ctx, (size_t)bytes / 2);   // two halves of bytes each
     if (rc) return rc;
     const size_t n = (size_t)bytes / 16;
@@ -103,6 +105,7 @@ int pmp_debug_epochs(pmp_ctx* ctx, int32_t mode) {
 int pmp_debug_epoch_report(pmp_ctx* ctx, int32_t* tiles, int32_t* violations, int32_t* min_epoch,
                            int32_t* max_epoch) {
     if (!ctx || !tiles || !violations || !min_epoch || !max_epoch) return PMP_EINVAL;
+    PMP_ON_DEVICE(ctx);
     if (!ctx->epochs || ctx->epoch_tiles.empty())
         return set_error(ctx, PMP_ESTATE, "no red-black pass has run with pass epochs on");
     std::vector<unsigned int> h(ctx->epoch_words);
@@ -133,6 +136,7 @@ int pmp_debug_epoch_report(pmp_ctx* ctx, int32_t* tiles, int32_t* violations, in
 int pmp_debug_refine_normals(pmp_ctx* ctx, const pmp_params* params, int32_t width, int32_t i, int32_t j, int32_t it,
                              int32_t n, float* out) {
     if (!ctx || !params || !out || n < 0 || n > 3 * kMaxRefine || width < 1) return PMP_EINVAL;
+    PMP_ON_DEVICE(ctx);
     int rc = ensure_dbg(ctx, 3 * kMaxRefine);
     if (rc) return rc;
     KfArgs a;

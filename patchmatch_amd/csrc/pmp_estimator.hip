@@ -192,6 +192,7 @@ extern "C" {
 int pmp_set_views(pmp_ctx* ctx, uint64_t ref_id, const pmp_pose* ref_pose, int32_t n_views, const uint64_t* view_ids,
                   const pmp_pose* view_poses) {
     if (!ctx || !ref_pose || (n_views > 0 && (!view_ids || !view_poses))) return PMP_EINVAL;
+    PMP_ON_DEVICE(ctx);
     if (n_views < 1 || n_views > kMaxViews) return set_error(ctx, PMP_EINVAL, "n_views must be in [1, 10]");
     auto it = ctx->images.find(ref_id);
     if (it == ctx->images.end()) return set_error(ctx, PMP_ENOTFOUND, "unknown reference image id");
@@ -229,6 +230,7 @@ int pmp_set_mask(pmp_ctx* ctx, int64_t mask_id) {
 int pmp_random_initialization(pmp_ctx* ctx, const pmp_params* params) {
     TraceRange trace_range("pmp_random_initialization");
     if (!ctx || !params) return PMP_EINVAL;
+    PMP_ON_DEVICE(ctx);
     QueueState* q = nullptr;
     QueueRun r;
     int rc = plain_prepare(ctx, params, nullptr, false, 0, false, q, r);
@@ -242,6 +244,7 @@ int pmp_random_initialization(pmp_ctx* ctx, const pmp_params* params) {
 int pmp_score_plane_depth(pmp_ctx* ctx, const pmp_params* params) {
     TraceRange trace_range("pmp_score_plane_depth");
     if (!ctx || !params) return PMP_EINVAL;
+    PMP_ON_DEVICE(ctx);
     QueueState* q = nullptr;
     QueueRun r;
     int rc = plain_prepare(ctx, params, nullptr, false, 0, false, q, r);
@@ -252,6 +255,7 @@ int pmp_score_plane_depth(pmp_ctx* ctx, const pmp_params* params) {
 int pmp_red_black_pass(pmp_ctx* ctx, const pmp_params* params, int32_t it_begin, int32_t it_end) {
     TraceRange trace_range("pmp_red_black_pass");
     if (!ctx || !params || it_begin < 0 || it_end < it_begin) return PMP_EINVAL;
+    PMP_ON_DEVICE(ctx);
     if (it_end - it_begin > kMaxPasses / 2) return set_error(ctx, PMP_EINVAL, "at most 64 iterations per call");
     QueueState* q = nullptr;
     QueueRun r;
@@ -271,6 +275,7 @@ int pmp_red_black_pass(pmp_ctx* ctx, const pmp_params* params, int32_t it_begin,
 int pmp_finalize(pmp_ctx* ctx, const pmp_params* params, const uint8_t* mask) {
     TraceRange trace_range("pmp_finalize");
     if (!ctx || !params) return PMP_EINVAL;
+    PMP_ON_DEVICE(ctx);
     if (!ctx->views_set) return set_error(ctx, PMP_ESTATE, "pmp_set_views has not been called");
     const uint8_t* dmask = nullptr;
     int rc = stage_mask(ctx, mask, dmask);
@@ -287,6 +292,7 @@ int pmp_finalize(pmp_ctx* ctx, const pmp_params* params, const uint8_t* mask) {
 int pmp_densify(pmp_ctx* ctx, const pmp_params* params, const uint8_t* mask) {
     TraceRange trace_range("pmp_densify");
     if (!ctx || !params || params->iterations < 0) return PMP_EINVAL;
+    PMP_ON_DEVICE(ctx);
     if (!ctx->views_set) return set_error(ctx, PMP_ESTATE, "pmp_set_views has not been called");
     const uint8_t* dmask = nullptr;
     int rc = stage_mask(ctx, mask, dmask);
@@ -301,6 +307,7 @@ int pmp_densify(pmp_ctx* ctx, const pmp_params* params, const uint8_t* mask) {
 
 int pmp_set_maps(pmp_ctx* ctx, const float* plane4, const float* depth, const float* score) {
     if (!ctx) return PMP_EINVAL;
+    PMP_ON_DEVICE(ctx);
     if (!ctx->views_set) return set_error(ctx, PMP_ESTATE, "pmp_set_views has not been called");
     const size_t P = (size_t)ctx->W * ctx->H;
     if (plane4) PMP_HIP(ctx, hipMemcpyAsync(ctx->ws.plane, plane4, P * 16, hipMemcpyHostToDevice, ctx->stream));
@@ -312,6 +319,7 @@ int pmp_set_maps(pmp_ctx* ctx, const float* plane4, const float* depth, const fl
 
 int pmp_get_maps(pmp_ctx* ctx, int32_t which, float* plane4, float* depth, float* score, int32_t* nghbr) {
     if (!ctx) return PMP_EINVAL;
+    PMP_ON_DEVICE(ctx);
     if (!ctx->views_set) return set_error(ctx, PMP_ESTATE, "pmp_set_views has not been called");
     const size_t P = (size_t)ctx->W * ctx->H;
     const void *sp, *sd, *ss;
@@ -332,6 +340,7 @@ int pmp_get_maps(pmp_ctx* ctx, int32_t which, float* plane4, float* depth, float
 
 int pmp_get_normals(pmp_ctx* ctx, int32_t which, float* normal3) {
     if (!ctx || !normal3) return PMP_EINVAL;
+    PMP_ON_DEVICE(ctx);
     if (!ctx->views_set) return set_error(ctx, PMP_ESTATE, "pmp_set_views has not been called");
     const size_t P = (size_t)ctx->W * ctx->H;
     const float4* sp;
@@ -355,6 +364,7 @@ int pmp_get_normals(pmp_ctx* ctx, int32_t which, float* normal3) {
 
 int pmp_eval_counts(pmp_ctx* ctx, int64_t* evals, int64_t* candidates) {
     if (!ctx) return PMP_EINVAL;
+    PMP_ON_DEVICE(ctx);
     unsigned long long h[2] = {0, 0};
     if (ctx->counters) {
         PMP_HIP(ctx, hipMemcpyAsync(h, ctx->counters, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));

@@ -332,6 +332,7 @@ static int clean_impl(pmp_ctx* ctx, int32_t width, int32_t height, int32_t n_map
                       const float* const* plane4_maps, const pmp_pose* poses, float same_depth_thr,
                       int32_t min_consistent, float max_reproj_px, float max_normal_deg, float* cleaned) {
     if (!ctx || !depth_maps || !poses || !cleaned || width < 2 || height < 2) return PMP_EINVAL;
+    PMP_ON_DEVICE(ctx);
     if (n_maps < 1 || n_maps > kMaxMaps) return set_error(ctx, PMP_EINVAL, "n_maps must be in [1, 16]");
     if (isnan(max_reproj_px) || !(max_normal_deg < 90.0f))
         return set_error(ctx, PMP_EINVAL, "max_reproj_px must not be NaN; max_normal_deg must be below 90");
@@ -416,7 +417,7 @@ int pmp_clean_noalloc(pmp_ctx* ctx, int32_t width, int32_t height, int32_t n_map
                          "2 * (width + height)); use pmp_clean");
     for (int o = 0; o < n_maps; ++o)
         if (!depth_maps[o]) return set_error(ctx, PMP_EINVAL, "null depth map");
-    PMP_HIP(ctx, hipSetDevice(ctx->device));
+    PMP_ON_DEVICE(ctx);
     float* base = nullptr;
     int rc = clean_scratch(ctx, base);
     if (rc) return rc;
@@ -460,6 +461,7 @@ int pmp_clean_noalloc(pmp_ctx* ctx, int32_t width, int32_t height, int32_t n_map
 int pmp_pruner_add_view(pmp_ctx* ctx, const float* cleaned_depth, const pmp_pose* pose, const uint8_t* bgr,
                         int32_t width, int32_t height) {
     if (!ctx || !cleaned_depth || !pose || !bgr || width < 2 || height < 2) return PMP_EINVAL;
+    PMP_ON_DEVICE(ctx);
     if (!ctx->pruner.empty() && (ctx->pruner[0].W != width || ctx->pruner[0].H != height))
         return set_error(ctx, PMP_EINVAL, "all pruner views must have the same size");
     if ((int)ctx->pruner.size() >= kMaxMaps) return set_error(ctx, PMP_EINVAL, "pruner queue is full (16)");
@@ -486,6 +488,7 @@ int pmp_pruner_prune(pmp_ctx* ctx, float same_depth_thr, float* pruned_depth, in
                      int32_t* kps, int64_t* count) {
     TraceRange trace_range("pmp_pruner_prune");
     if (!ctx || cap < 0 || (cap > 0 && (!pts || !cols || !kps))) return PMP_EINVAL;
+    PMP_ON_DEVICE(ctx);
     if (ctx->pruner.empty()) {
         if (count) *count = 0;
         return PMP_OK;
@@ -565,6 +568,7 @@ int pmp_pruner_prune(pmp_ctx* ctx, float same_depth_thr, float* pruned_depth, in
 
 int pmp_pruner_pop_head(pmp_ctx* ctx) {
     if (!ctx) return PMP_EINVAL;
+    PMP_ON_DEVICE(ctx);
     if (ctx->pruner.empty()) return PMP_OK;
     PMP_HIP(ctx, hipStreamSynchronize(ctx->stream));
     (void)hipFree(ctx->pruner[0].depth);
@@ -575,6 +579,7 @@ int pmp_pruner_pop_head(pmp_ctx* ctx) {
 
 int pmp_pruner_reset(pmp_ctx* ctx) {
     if (!ctx) return PMP_EINVAL;
+    PMP_ON_DEVICE(ctx);
     PMP_HIP(ctx, hipStreamSynchronize(ctx->stream));
     for (auto& v : ctx->pruner) {
         (void)hipFree(v.depth);
@@ -588,6 +593,7 @@ int pmp_pruner_size(const pmp_ctx* ctx) { return ctx ? (int)ctx->pruner.size() :
 
 int pmp_pruner_get_map(pmp_ctx* ctx, int32_t index, float* depth) {
     if (!ctx || !depth) return PMP_EINVAL;
+    PMP_ON_DEVICE(ctx);
     if (index < 0 || index >= (int)ctx->pruner.size()) return set_error(ctx, PMP_EINVAL, "pruner index out of range");
     const PrunerView& v = ctx->pruner[index];
     PMP_HIP(ctx, hipMemcpyAsync(depth, v.depth, (size_t)v.W * v.H * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));

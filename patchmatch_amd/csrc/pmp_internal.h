@@ -220,3 +220,39 @@ int build_args(pmp_ctx* ctx, const pmp_params* p, uint64_t ref_id, int N, const 
         hipError_t e_ = (expr);                                     \
         if (e_ != hipSuccess) return pmp::check_hip(ctx, e_, #expr); \
     } while (0)
+
+namespace pmp {
+
+// The device of a context made current on the calling thread for the length
+// of a call, and the caller's own current device put back when it returns:
+// allocations, copies and launches land on the context's device from any
+// thread, and contexts on different GPUs can be driven from one thread.
+class DeviceScope {
+  public:
+    explicit DeviceScope(int device) {
+        int cur = -1;
+        if (hipGetDevice(&cur) != hipSuccess) return;
+        if (cur == device) { ok_ = true; return; }
+        if (hipSetDevice(device) != hipSuccess) return;
+        prev_ = cur;
+        ok_ = true;
+    }
+    ~DeviceScope() {
+        if (prev_ >= 0) (void)hipSetDevice(prev_);
+    }
+    DeviceScope(const DeviceScope&) = delete;
+    DeviceScope& operator=(const DeviceScope&) = delete;
+    bool ok() const { return ok_; }
+
+  private:
+    int prev_ = -1;
+    bool ok_ = false;
+};
+
+}  // namespace pmp
+
+// First statement of an entry point after its argument checks.
+#define PMP_ON_DEVICE(ctx)                                                  \
+    pmp::DeviceScope pmp_device_scope_((ctx)->device);                      \
+    if (!pmp_device_scope_.ok())                                            \
+        return pmp::set_error((ctx), PMP_EDEVICE, "cannot make the context's device current")

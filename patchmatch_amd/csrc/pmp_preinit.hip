@@ -335,6 +335,7 @@ int pmp_preinit_landmarks(pmp_ctx* ctx, const pmp_params* params, int64_t n, con
                           int64_t* accepted) {
     TraceRange trace_range("pmp_preinit_landmarks");
     if (!ctx || !params || n < 0 || (n > 0 && !xyz)) return PMP_EINVAL;
+    PMP_ON_DEVICE(ctx);
     if (n > 0x7FFFFFFF) return set_error(ctx, PMP_EINVAL, "too many landmarks");
     if (accepted) *accepted = 0;
     if (!ctx->views_set) return set_error(ctx, PMP_ESTATE, "pmp_set_views has not been called");
@@ -360,6 +361,7 @@ int pmp_preinit_landmarks(pmp_ctx* ctx, const pmp_params* params, int64_t n, con
 
 int pmp_memorize_result(pmp_ctx* ctx) {
     if (!ctx) return PMP_EINVAL;
+    PMP_ON_DEVICE(ctx);
     if (!ctx->views_set) return set_error(ctx, PMP_ESTATE, "pmp_set_views has not been called");
     PreState* s = nullptr;
     int rc = pre_state(ctx, s);
@@ -388,6 +390,7 @@ int pmp_forget_result(pmp_ctx* ctx) {
 int pmp_preinit_warp(pmp_ctx* ctx, const pmp_params* params, const uint8_t* mask, int64_t* accepted) {
     TraceRange trace_range("pmp_preinit_warp");
     if (!ctx || !params) return PMP_EINVAL;
+    PMP_ON_DEVICE(ctx);
     if (accepted) *accepted = 0;
     if (!ctx->views_set) return set_error(ctx, PMP_ESTATE, "pmp_set_views has not been called");
     PreState* s = ctx->pre;

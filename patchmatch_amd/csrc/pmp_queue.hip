@@ -555,7 +555,7 @@ int pmp_batch_densify(pmp_ctx* ctx, const pmp_params* params, int32_t n, const p
     TraceRange trace_range("pmp_batch_densify");
     if (!ctx || !params || !kfs) return PMP_EINVAL;
     if (n < 1 || n > kMaxSlots) return set_error(ctx, PMP_EINVAL, "batch size must be in [1, PMP_MAX_BATCH]");
-    PMP_HIP(ctx, hipSetDevice(ctx->device));
+    PMP_ON_DEVICE(ctx);
     QueueState* q = nullptr;
     int rc = queue_state(ctx, q);
     if (rc) return rc;
@@ -571,6 +571,7 @@ int pmp_batch_densify(pmp_ctx* ctx, const pmp_params* params, int32_t n, const p
 
 int pmp_batch_get_maps(pmp_ctx* ctx, int32_t k, float* plane4, float* depth, float* score, int32_t* nghbr) {
     if (!ctx) return PMP_EINVAL;
+    PMP_ON_DEVICE(ctx);
     QueueState* q = ctx->queue;
     if (!q || k < 0 || k >= q->n_last) return set_error(ctx, PMP_EINVAL, "no such keyframe in the last batch");
     const SlotBuf& b = q->bufs[k];
@@ -585,6 +586,7 @@ int pmp_batch_get_maps(pmp_ctx* ctx, int32_t k, float* plane4, float* depth, flo
 
 int pmp_batch_set_config(pmp_ctx* ctx, const pmp_batch_config* cfg) {
     if (!ctx || !cfg) return PMP_EINVAL;
+    PMP_ON_DEVICE(ctx);
     int rc = check_config(ctx, *cfg);
     if (rc) return rc;
     QueueState* q = nullptr;
@@ -595,6 +597,7 @@ int pmp_batch_set_config(pmp_ctx* ctx, const pmp_batch_config* cfg) {
 
 int pmp_batch_get_config(pmp_ctx* ctx, pmp_batch_config* cfg) {
     if (!ctx || !cfg) return PMP_EINVAL;
+    PMP_ON_DEVICE(ctx);
     QueueState* q = nullptr;
     int rc = queue_state(ctx, q);
     if (rc) return rc;
@@ -607,7 +610,7 @@ int pmp_batch_autotune(pmp_ctx* ctx, const pmp_params* params, int32_t n, const 
     TraceRange trace_range("pmp_batch_autotune");
     if (!ctx || !params || !kfs) return PMP_EINVAL;
     if (n < 1 || n > kMaxSlots) return set_error(ctx, PMP_EINVAL, "batch size must be in [1, PMP_MAX_BATCH]");
-    PMP_HIP(ctx, hipSetDevice(ctx->device));
+    PMP_ON_DEVICE(ctx);
     QueueState* q = nullptr;
     int rc = queue_state(ctx, q);
     if (rc) return rc;

@@ -35,7 +35,9 @@ class GpuExec final : public sched::Executor {
     pmp_batch_config cfg{8, 0, 1, 1};
     int edge_mode = PMP_EDGE_WRAP;
     // ctx->images and ctx->last_error: touched by the submitting thread
-    // (upload, pack) and the scheduler thread (drop, slot_grow, step_issue)
+    // (upload, pack) and the scheduler thread (drop, slot_grow, step_issue).
+    // The scheduler thread keeps the context's device current; calls on the
+    // caller's threads (submit, reset) hold a DeviceScope.
     std::mutex pool_mu;
     // pinned staging of uploads (one copy from pinned memory instead of a
     // pageable one), shared by the submitting threads
@@ -48,6 +50,7 @@ class GpuExec final : public sched::Executor {
         edge_mode = o.edge_mode;
         int rc = pmp_create(&ctx, o.device);
         if (rc) return rc;
+        DeviceScope dev(ctx->device);
         QueueState* q = nullptr;
         rc = queue_state(ctx, q);
         if (!rc) rc = desc_alloc(ctx, desc[0], kMaxSlots);
@@ -65,7 +68,7 @@ class GpuExec final : public sched::Executor {
 
     ~GpuExec() override {
         if (!ctx) return;
-        (void)hipSetDevice(ctx->device);
+        DeviceScope dev(ctx->device);
         (void)hipStreamSynchronize(ctx->stream);
         for (auto& b : slots) free_slot(b);
         desc_free(desc[0]);
@@ -97,7 +100,6 @@ class GpuExec final : public sched::Executor {
 
     int upload(uint64_t id, const uint8_t* pixels, int W, int H, std::string& err) override {
         TraceRange trace_range("pmp_server upload");
-        (void)hipSetDevice(ctx->device);
         Image im;
         im.W = W;
         im.H = H;
@@ -130,7 +132,6 @@ class GpuExec final : public sched::Executor {
     }
 
     int pack(uint64_t id, int wrap, std::string& err) override {
-        (void)hipSetDevice(ctx->device);
         std::lock_guard<std::mutex> lk(pool_mu);   // ensure_quads reports through the context
         auto it = ctx->images.find(id);
         if (it == ctx->images.end()) {
@@ -144,7 +145,6 @@ class GpuExec final : public sched::Executor {
     }
 
     void drop(uint64_t id) override {
-        (void)hipSetDevice(ctx->device);   // also called from pmp_server_reset's thread
         std::lock_guard<std::mutex> lk(pool_mu);
         auto it = ctx->images.find(id);
         if (it == ctx->images.end()) return;
@@ -326,7 +326,7 @@ const char* pmp_server_last_error(pmp_server* s) {
 
 int pmp_server_submit(pmp_server* s, const pmp_request* req, const pmp_params* params, uint64_t* ticket) {
     if (!s || !req || !params || !ticket) return PMP_EINVAL;
-    (void)hipSetDevice(s->exec.ctx->device);
+    DeviceScope dev(s->exec.ctx->device);   // uploads and packing run on this thread
     std::string err;
     const int rc = s->srv->submit(*req, *params, ticket, err);
     return rc ? fail(rc, err) : PMP_OK;
@@ -356,6 +356,7 @@ int pmp_server_cancel(pmp_server* s, uint64_t ticket) {
 
 int pmp_server_reset(pmp_server* s) {
     if (!s) return PMP_EINVAL;
+    DeviceScope dev(s->exec.ctx->device);   // the pool's images are freed on this thread
     std::string err;
     const int rc = s->srv->reset(err);
     return rc ? fail(rc, err) : PMP_OK;

@@ -139,7 +139,7 @@ int pmp_resize(pmp_ctx* ctx, const uint8_t* src, int32_t width, int32_t height, 
     if (!ctx || !src || !dst) return PMP_EINVAL;
     int rc = check_dims(ctx, width, height, pitch, channels, out_width, out_height);
     if (rc) return rc;
-    PMP_HIP(ctx, hipSetDevice(ctx->device));
+    PMP_ON_DEVICE(ctx);
     const uint8_t* dsrc;
     int64_t dpitch;
     if ((rc = stage_src(ctx, src, width, height, pitch, channels, on_device != 0, dsrc, dpitch))) return rc;
@@ -167,7 +167,7 @@ int pmp_image_put_thumbnail(pmp_ctx* ctx, uint64_t id, const uint8_t* pixels, in
     if (rc) return rc;
     if (color_order < PMP_COLOR_GRAY || color_order > PMP_COLOR_BGR)
         return set_error(ctx, PMP_EINVAL, "unknown color order");
-    PMP_HIP(ctx, hipSetDevice(ctx->device));
+    PMP_ON_DEVICE(ctx);
     const uint8_t* dsrc;
     int64_t dpitch;
     if ((rc = stage_src(ctx, pixels, width, height, pitch, channels, on_device != 0, dsrc, dpitch))) return rc;
