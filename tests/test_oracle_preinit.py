@@ -133,3 +133,82 @@ def test_warp_respects_masks_and_depth_floor(kf_c0):
     assert np.all(np.isnan(sc[:, : kf_c0.W // 2]))
     assert np.all(np.isnan(sc[100:140]))
     assert np.all(w2[2][100:140] == -2.0)
+
+
+def test_warp_under_yaw_keeps_unrotated_normal_and_masks_the_source(kf_c0):
+    """PreInitResultWarp (depthmap.cc:444-486) under a pure yaw of the camera:
+    a previous result whose camera is the current one turned by 30 degrees
+    (40 of 480 columns) about its vertical axis, of the same surface.  Every
+    point keeps its distance and moves 40 columns, so each candidate lands at
+    a known pixel with the surface's depth.  Two quirks are then visible:
+      * the candidate's normal is the previous plane's direction as stored,
+        in the previous camera's frame, not rotated into the current one
+        (the rotating line is commented out at :472): it differs from the
+        surface's normal by the 30-degree turn wherever that normal is
+        horizontal;
+      * the current mask is tested at the previous pixel (:444) as well as at
+        the target (:463): a masked column band blocks the sources inside it
+        whose targets lie outside, not only the targets inside it."""
+    kf = kf_c0
+    W, H = kf.W, kf.H
+    p = config0_params()
+    okf, work, init = _setup(kf, p)
+    shift = 40
+    th = 2 * np.pi * shift / W
+    Ry = np.array([[np.cos(th), 0.0, np.sin(th)], [0.0, 1.0, 0.0], [-np.sin(th), 0.0, np.cos(th)]])
+    B = synth.bearings(W, H)
+    # the pixel (r, c) of the previous camera looks along the current pixel (r, (c + s) % W)
+    d_cur = B @ Ry                                   # rows: Ry^T B[r, c]
+    s = None
+    for cand in (shift, -shift):
+        if np.abs(np.roll(B, -cand, 1) - d_cur).max() < 1e-9:
+            s = cand
+    assert s is not None
+    tgt_col = (np.arange(W) + s) % W
+    gt = gt_planes(kf)
+    prev = np.zeros_like(gt)
+    prev[..., :3] = (gt[:, tgt_col, :3].astype(np.float64) @ Ry.T).astype(np.float32)   # p_prev = Ry p_cur
+    pdepth = kf.gt_depth[:, tgt_col].astype(np.float32)
+    R = kf.ref_pose[:9].reshape(3, 3)
+    t = kf.ref_pose[9:]
+    prev_pose = np.concatenate([(Ry @ R).ravel(), Ry @ t])
+    mask = np.ones((H, W), np.uint8)
+    band = (W // 4, W // 2)
+    mask[:, band[0]:band[1]] = 0
+    low = tuple(a.copy() for a in work)
+    low[2][:] = -2.0                                 # every candidate beats this
+    w2, _, sc, acc = or_py.preinit_warp(okf, p, low, init, mask, prev_pose, prev, pdepth)
+    cols = np.arange(W)
+    src_in = (cols >= band[0]) & (cols < band[1])
+    tgt_in = (tgt_col >= band[0]) & (tgt_col < band[1])
+    # sources inside the band never form a candidate, also where their target is outside it
+    assert np.all(np.isnan(sc[:, src_in]))
+    assert (src_in & ~tgt_in).sum() >= shift - 2
+    # targets inside the band are blocked too
+    assert np.all(np.isnan(sc[:, tgt_in]))
+    # elsewhere the candidates are formed (interior rows, targets inside the scored grid and
+    # clear of the band's edges and of rounding at the 40-column seam)
+    ok_cols = ~src_in & ~tgt_in & (tgt_col >= 10) & (tgt_col < 440)
+    for e in band:
+        ok_cols &= np.abs(tgt_col - e) > 2
+    ok_cols &= np.abs(cols - band[0]) > 2
+    ok_cols &= np.abs(cols - band[1]) > 2
+    assert (~np.isnan(sc[10:210][:, ok_cols])).mean() > 0.85
+    # accepted candidates sit at their target pixel with the surface's depth
+    moved = w2[2] > -2.0
+    assert acc == moved.sum() > 1000
+    r, c = np.nonzero(moved)
+    assert np.all(mask[r, c] == 1)
+    np.testing.assert_allclose(w2[1][r, c], kf.gt_depth[r, c], rtol=2e-4)
+    # and carry the previous frame's normal: Ry n, not n
+    n_cur = gt[r, c, :3].astype(np.float64)
+    n_cur /= np.linalg.norm(n_cur, axis=-1, keepdims=True)
+    n_prev = n_cur @ Ry.T
+    got = w2[0][r, c, :3].astype(np.float64)
+    got /= np.linalg.norm(got, axis=-1, keepdims=True)
+    horiz = np.abs(n_cur[:, 1]) < 0.3
+    assert horiz.sum() > 500
+    cos_prev = np.abs(np.sum(got * n_prev, -1))
+    cos_cur = np.abs(np.sum(got * n_cur, -1))
+    assert np.all(cos_prev[horiz] > 1 - 1e-6)
+    assert np.all(cos_cur[horiz] < np.cos(np.radians(25)))
