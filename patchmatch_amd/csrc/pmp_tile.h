@@ -58,31 +58,87 @@ __device__ __forceinline__ void init_pixel(const KfArgs& a, const float* sWc, in
 // are at odd distance (1 or 3), so they have the other colour and are not
 // written by this pass.
 //
-// Propagation candidates that survive the exact pre-rejection of a pixel are
-// scored by whichever threads of the block are free: per candidate index the
-// surviving (pixel, plane) jobs are packed into the first waves through LDS,
-// scored there, and the raw scores handed back to the owning threads, which
-// apply the penalty and the acceptance test in candidate order.  A pixel's
-// sequence of decisions is unchanged; only the lanes that compute its scores
-// differ, so results are identical to the one-thread-per-pixel loop.
+// Propagation candidates are scored by whichever threads of the block are
+// free, in packed sweeps, and their raw scores handed back to the owning
+// threads, which apply the penalty and the acceptance test in candidate order:
+//   1. every pixel tests its 8 candidates against its state at entry (the
+//      exact pre-rejection below); the (candidate, pixel) pairs that pass are
+//      listed candidate-major and scored in one sweep over the block's threads;
+//   2. each pixel then walks its candidates in order against its running state
+//      (the test is repeated: an acceptance changes the plane and the score it
+//      is tested against).  A candidate that passes there but did not pass at
+//      entry has no score yet: the pixel stops and lists every later candidate
+//      that can still pass at all (1 - f_d above its current score: the plane
+//      factor is at most 1 and the score only grows), and a second sweep scores
+//      those of all such pixels before the walks go on.  After it no walk
+//      stops again, so a tile takes at most two sweeps.
+// A candidate's raw score depends only on its plane and the pixel, so where
+// and when it is computed does not matter: each pixel's sequence of decisions
+// is that of the one-thread-per-pixel loop, and the results are identical.
+// Scores computed for candidates that an acceptance makes unnecessary are
+// spent work (counted as evaluations).
 // NVS: per-view score scratch rows (kMaxViews for the run-time view count, 1
 // otherwise).
 // srow/scol/sref: the tile's rows and columns of the bearing tables and of
 // the reference image, with the patch halo (LdsTaps).
+// res[c * 64R + t]: raw score of candidate c of thread t's pixel; job: the
+// sweep's (candidate << 9 | thread) list.
 template <int R, int NVS = 1>
 struct PropShared {
     float sWc[256];
     float sv[NVS][64 * R];
-    float4 job[64 * R];
+    float res[8 * 64 * R];
     float4 rs[64 * R];
     float ic[64 * R];
-    float res[64 * R];
+    uint16_t job[8 * 64 * R];
     float2 srow[64 * R / kTileSame + 2 * kMaxHalf];
     float2 scol[kTileCols];
     uint8_t sref[(64 * R / kTileSame + 2 * kMaxHalf) * kTileCols];
-    int cnt[R];
+    int cnt[8 * R];
     int item;
 };
+
+// Offset (di, dj) of propagation candidate c: (0,+1) (+1,0) (0,-1) (-1,0)
+// (0,+3) (+3,0) (0,-3) (-3,0) (PatchMatchCuda.cu:305)
+__device__ __forceinline__ void cand_offset(int c, int& di, int& dj) {
+    const int dist = (c < 4) ? 1 : 3, dir = c & 3;
+    di = (dir == 1) ? dist : (dir == 3) ? -dist : 0;
+    dj = (dir == 0) ? dist : (dir == 2) ? -dist : 0;
+}
+
+// 1 - f_d of candidate plane pn: the largest penalty factor it can have for a
+// pixel with bearing b and entry depth d0 (PatchMatchCuda.cu:323-326), whatever
+// the running plane.
+__device__ __forceinline__ float cand_bound(const KfArgs& a, float4 pn, float bx, float by, float bz, float d0,
+                                           float inv_d0) {
+    const float s = fma_(pn.z, bz, fma_(pn.y, by, pn.x * bx));
+    const float dn = rcp_pos(clampf(abs_(s), 1e-6f, 1e30f));
+    const float rel = (d0 - dn) * inv_d0;
+    const float fd = exp_((rel * rel) * -a.depth_penalty);
+    return 1.0f - fd;
+}
+
+// Penalty and acceptance precondition of candidate plane pn for a pixel with
+// bearing b, entry depth d0 and running plane h and score sc
+// (PatchMatchCuda.cu:323-332).  acosf(prod/norm) is NaN, i.e. the candidate
+// is rejected, unless |prod| <= norm; |score| <= 1, so the penalised score
+// cannot exceed f, and f <= sc rejects it before it is scored.
+__device__ __forceinline__ bool cand_test(const KfArgs& a, float4 pn, float bx, float by, float bz, float d0,
+                                          float inv_d0, float h0, float h1, float h2, float sc, float& f) {
+    const float gd = cand_bound(a, pn, bx, by, bz, d0, inv_d0);
+    const float prod = fma_(h2, pn.z, fma_(h1, pn.y, h0 * pn.x));
+    const float ex = h0 - pn.x, ey = h1 - pn.y, ez = h2 - pn.z;
+    const float nrm = sqrt_(fma_(ez, ez, fma_(ey, ey, ex * ex)));
+    f = 0.0f;
+    if (nrm > 0.0f && abs_(prod) <= nrm) {
+        const float ratio = clampf(prod * rcp_pos(clampf(nrm, 1e-30f, 1e30f)), -1.0f, 1.0f);
+        const float ac = acos_(ratio);
+        const float fp = exp_((ac * ac) * -a.plane_penalty);
+        f = gd * (1.0f - fp);
+        return f > sc;
+    }
+    return false;
+}
 
 // A propagation tile is TR = 64 R / kTileSame rows by 2 kTileSame columns
 // (kTileSame same-colour pixels per row; each wave covers 64 / kTileSame
@@ -151,67 +207,92 @@ __device__ __forceinline__ void propagate_tile(const KfArgs& a, PropShared<R, NV
     sh.rs[tid] = make_float4(rs.mx, rs.inv, rs.mxc, rs.vx);
     sh.ic[tid] = rs.Ic;
 
-    // 1. propagation (PatchMatchCuda.cu:311-338)
-    // candidate order (0,+1) (+1,0) (0,-1) (-1,0) (0,+3) (+3,0) (0,-3) (-3,0)
+    // 1. propagation (PatchMatchCuda.cu:311-338), in the order of cand_offset
+    unsigned int want = 0;     // bit c: candidate c is to be scored in the next sweep
+    if (active) {
+        n_cand += 8;
 #pragma unroll 1
-    for (int c = 0; c < 8; ++c) {
-        bool need = false;
-        float4 pn = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        float f = 0.0f;
-        if (active) {
-            const int dist = (c < 4) ? 1 : 3, dir = c & 3;
-            const int di = (dir == 1) ? dist : (dir == 3) ? -dist : 0;
-            const int dj = (dir == 0) ? dist : (dir == 2) ? -dist : 0;
-            pn = plane[(size_t)(i + di) * W + (j + dj)];
-            ++n_cand;
-            const float s = fma_(pn.z, bz, fma_(pn.y, by, pn.x * bx));
-            const float dn = rcp_pos(clampf(abs_(s), 1e-6f, 1e30f));
-            const float rel = (d0 - dn) * inv_d0;
-            const float fd = exp_((rel * rel) * -a.depth_penalty);
-            const float prod = fma_(h2, pn.z, fma_(h1, pn.y, h0 * pn.x));
-            const float ex = h0 - pn.x, ey = h1 - pn.y, ez = h2 - pn.z;
-            const float nrm = sqrt_(fma_(ez, ez, fma_(ey, ey, ex * ex)));
-            // acosf(prod/norm) is NaN, i.e. the candidate is rejected, unless |prod| <= norm
-            if (nrm > 0.0f && abs_(prod) <= nrm) {
-                const float ratio = clampf(prod * rcp_pos(clampf(nrm, 1e-30f, 1e30f)), -1.0f, 1.0f);
-                const float ac = acos_(ratio);
-                const float fp = exp_((ac * ac) * -a.plane_penalty);
-                f = (1.0f - fd) * (1.0f - fp);
-                // |score| <= 1: the penalised score cannot exceed f
-                need = f > sc;
-            }
+        for (int c = 0; c < 8; ++c) {
+            int di, dj;
+            cand_offset(c, di, dj);
+            const float4 pn = plane[(size_t)(i + di) * W + (j + dj)];
+            float f;
+            if (cand_test(a, pn, bx, by, bz, d0, inv_d0, h0, h1, h2, sc, f)) want |= 1u << c;
         }
-        // pack the jobs of this candidate index
-        const unsigned long long bal = __ballot(need);
-        if (lane == 0) sh.cnt[wave] = __popcll(bal);
+    }
+    unsigned int listed = want;   // bit c: candidate c has a score (or will after the sweep)
+    int cpos = 0;                 // the next candidate of this pixel's walk
+#pragma unroll 1
+    for (;;) {
+        // the sweep's list, candidate-major: per (candidate, wave) counts, then offsets
+#pragma unroll 1
+        for (int c = 0; c < 8; ++c) {
+            const unsigned long long bal = __ballot((want >> c) & 1u);
+            if (lane == 0) sh.cnt[c * R + wave] = __popcll(bal);
+        }
         __syncthreads();
-        int off = 0, total = 0;
+        int J = 0;
+#pragma unroll 1
+        for (int c = 0; c < 8; ++c) {
+            int woff = 0, tot = 0;
 #pragma unroll
-        for (int w = 0; w < R; ++w) {
-            const int cw = sh.cnt[w];
-            off += (w < wave) ? cw : 0;
-            total += cw;
+            for (int w = 0; w < R; ++w) {
+                const int cw = sh.cnt[c * R + w];
+                woff += (w < wave) ? cw : 0;
+                tot += cw;
+            }
+            const bool on = (want >> c) & 1u;
+            const unsigned long long bal = __ballot(on);
+            if (on) sh.job[J + woff + __popcll(bal & ((1ull << lane) - 1ull))] = (uint16_t)((c << 9) | tid);
+            J += tot;
         }
-        if (need) {
-            const int slot = off + __popcll(bal & ((1ull << lane) - 1ull));
-            sh.job[(slot + rot) & (NT - 1)] = make_float4(pn.x, pn.y, pn.z, __int_as_float(tid));
-        }
+        if (J == 0) break;   // the same in every thread
         __syncthreads();
-        if (((tid - rot) & (NT - 1)) < total) {
-            const float4 jb = sh.job[tid];
-            const int jl = __float_as_int(jb.w);
+        // score the list: job g by thread (g + rot) mod NT
+#pragma unroll 1
+        for (int g = (tid - rot) & (NT - 1); g < J; g += NT) {
+            const unsigned int e = sh.job[g];
+            const int c = (int)(e >> 9), jl = (int)(e & 511u);
             const int il = tby * TR + jl / kTileSame;
             const int jc = 2 * (tbx * kTileSame + jl % kTileSame) + ((il + colour) & 1);
+            int di, dj;
+            cand_offset(c, di, dj);
+            const float4 pn = plane[(size_t)(il + di) * W + (jc + dj)];
             const float4 q = sh.rs[jl];
             RefStat r2;
             r2.Ic = sh.ic[jl]; r2.mx = q.x; r2.inv = q.y; r2.mxc = q.z; r2.vx = q.w;
-            sh.res[jl] = score_plane_t<NV, kZero>(a, tp, sWc, r2, il, jc, jb.x, jb.y, jb.z, &sh.sv[0][tid], NT);
+            sh.res[c * NT + jl] = score_plane_t<NV, kZero>(a, tp, sWc, r2, il, jc, pn.x, pn.y, pn.z, &sh.sv[0][tid], NT);
+            ++n_eval;
         }
         __syncthreads();
-        if (need) {
-            const float sh_ = sh.res[tid] * f;
-            ++n_eval;
-            if (sh_ > sc) { h0 = pn.x; h1 = pn.y; h2 = pn.z; sc = sh_; }
+        // walk the candidates in order against the running state, up to the
+        // first one that passes but has no score
+        want = 0;
+        if (active) {
+            while (cpos < 8) {
+                int di, dj;
+                cand_offset(cpos, di, dj);
+                const float4 pn = plane[(size_t)(i + di) * W + (j + dj)];
+                float f;
+                if (cand_test(a, pn, bx, by, bz, d0, inv_d0, h0, h1, h2, sc, f)) {
+                    if (!((listed >> cpos) & 1u)) break;
+                    const float sh_ = sh.res[cpos * NT + tid] * f;
+                    if (sh_ > sc) { h0 = pn.x; h1 = pn.y; h2 = pn.z; sc = sh_; }
+                }
+                ++cpos;
+            }
+            if (cpos < 8) {
+                // stopped: list this and every later unscored candidate that can still pass
+#pragma unroll 1
+                for (int c = cpos; c < 8; ++c) {
+                    if ((listed >> c) & 1u) continue;
+                    int di, dj;
+                    cand_offset(c, di, dj);
+                    const float4 pn = plane[(size_t)(i + di) * W + (j + dj)];
+                    if (c == cpos || cand_bound(a, pn, bx, by, bz, d0, inv_d0) > sc) want |= 1u << c;
+                }
+                listed |= want;
+            }
         }
     }
     if (active) {
