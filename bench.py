@@ -3,7 +3,8 @@
 Workload (the configuration the headline metric is quoted on): one
 1920x960 uint8 equirectangular reference keyframe + 6 neighbours (baselines
 0.05-0.5 m) rendered from a seeded synthetic scene (box room, 20 spheres,
-1.2 m tube, value-noise textures, photometric jitter, bottom 15 % masked);
+1.2 m tube, 4-octave value-noise textures of 4 to 32 cells per metre,
+photometric jitter, bottom 15 % masked);
 11x11 patch at stride 2 (36 taps), best-3-of-6 mean NCC, 8 red-black
 iterations, FP32.  A step is one full densification of that keyframe
 (random init + 16 red-black passes + copy-back + PostProcess) with images and
@@ -49,14 +50,14 @@ HBM_PEAK_GBS = 8000.0
 # L2-to-memory bytes per k_propagate_q launch on the 1920x960 + 6-view
 # workload, from rocprofv3 FETCH_SIZE and WRITE_SIZE scaled by a copy of known
 # size (profiles/k_propagate_q_c1_1920x960_6views.md); None for other workloads.
-TRAFFIC_BYTES_PER_LAUNCH_C1 = int((360_557 * 2.00 + 42_982 * 1.00) * 1024)
+TRAFFIC_BYTES_PER_LAUNCH_C1 = int((186_371 * 2.00 + 42_982 * 1.00) * 1024)
 TRAFFIC_SOURCE = ("profile-derived, not measured in this run: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE of "
                   "k_propagate_q on this workload, scaled by a streaming copy of known size "
                   "(profiles/k_propagate_q_c1_1920x960_6views.md)")
 # VALU wave-instructions per k_propagate_q launch on the 1920x960 + 6-view
 # workload (rocprofv3 --pmc SQ_INSTS_VALU, same profile): the issue side of
 # the roofline, against one wave64 VALU instruction per CU per clock.
-VALU_WAVE_INSTS_PER_LAUNCH_C1 = 3.137e9
+VALU_WAVE_INSTS_PER_LAUNCH_C1 = 2.509e9
 CUS, CLOCK_HZ = 256, 2.4e9
 
 
@@ -245,6 +246,7 @@ def main(argv=None):
                    else f"plain drop-in: 1 keyframe {args.width}x{args.height} + {N} neighbours"
                         + (f", params from {args.params}" if args.params else ""),
                    "width": args.width, "height": args.height, "views": N, "taps": T, "iterations": p.iterations,
+                   "texture_cells_per_m": [synth.TEX_FREQ_C1, 8 * synth.TEX_FREQ_C1],
                    "form": "plain", **({"params": p.as_dict()} if args.params else {})},
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3), "p99": round(float(np.percentile(lat, 99)), 3),
                        "max": round(float(np.max(lat)), 3)},

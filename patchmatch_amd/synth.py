@@ -19,6 +19,17 @@ import numpy as np
 ROOM_LO = np.array([-4.0, -1.5, -3.0])
 ROOM_HI = np.array([4.0, 1.5, 3.0])
 
+# Lattice frequency (cells per metre) of the texture's coarsest octave.  The
+# finest octave has 8 x as many cells, and each config's scene sizes them to
+# about 3 pixels at 3 m in its working panorama (one pixel there spans
+# 3 m x 2 pi / W): 1 cell per metre for the 480-pixel box room (config 0),
+# 4 for the 1920-pixel scene of configs 1-4.  Coarser texture starves the
+# 11 x 11 patch (at 1 cell per metre and 1920 pixels the finest cells span
+# 13 pixels, and the reference algorithm puts 15 % of the pixels within 1 %
+# of the true depth); finer texture aliases.
+TEX_FREQ_C0 = 1.0
+TEX_FREQ_C1 = 4.0
+
 
 def _split_seeds(seed: int, n: int) -> list[int]:
     """SplitMix64 stream of n 64-bit seeds from one master seed."""
@@ -264,7 +275,8 @@ def photometric(img: np.ndarray, rng: np.random.Generator) -> np.ndarray:
     return np.clip(np.rint(out), 0, 255).astype(np.uint8)
 
 
-def make_scene(seed: int, spheres: int = 0, tube: bool = False, cam_region: float = 0.8) -> Scene:
+def make_scene(seed: int, spheres: int = 0, tube: bool = False, cam_region: float = 0.8,
+               base_freq: float = TEX_FREQ_C0) -> Scene:
     s_scene, s_tex = _split_seeds(seed, 2)
     rng = np.random.default_rng(s_scene)
     sph = []
@@ -277,7 +289,7 @@ def make_scene(seed: int, spheres: int = 0, tube: bool = False, cam_region: floa
             continue
         sph.append([c[0], c[1], c[2], r])
     tb = (0.5, 2.0, 0.6, -1.5, 0.5) if tube else None
-    return Scene(np.array(sph, dtype=np.float64).reshape(-1, 4), tb, ValueNoise3(s_tex))
+    return Scene(np.array(sph, dtype=np.float64).reshape(-1, 4), tb, ValueNoise3(s_tex), base_freq)
 
 
 @dataclasses.dataclass
@@ -295,11 +307,12 @@ class Keyframe:
 
 
 def make_keyframe(seed: int, W: int, H: int, n_views: int, baseline=(0.1, 0.3),
-                  spheres: int = 0, tube: bool = False, mask_frac: float = 0.15, workers: int = 8) -> Keyframe:
+                  spheres: int = 0, tube: bool = False, mask_frac: float = 0.15, workers: int = 8,
+                  base_freq: float = TEX_FREQ_C0) -> Keyframe:
     """Reference camera near the room centre plus n_views neighbours at random
     baselines/yaws (seeded)."""
     s_scene, s_pose, s_photo = _split_seeds(seed ^ 0x5EED, 3)
-    scene = make_scene(s_scene, spheres=spheres, tube=tube)
+    scene = make_scene(s_scene, spheres=spheres, tube=tube, base_freq=base_freq)
     rng = np.random.default_rng(s_pose)
     c0 = np.array([rng.uniform(-0.2, 0.2), rng.uniform(-0.05, 0.05), rng.uniform(-0.2, 0.2)])
     ref_pose = yaw_pose(rng.uniform(-np.pi, np.pi), c0)
@@ -337,19 +350,21 @@ def config0(seed: int = 0) -> Keyframe:
     return make_keyframe(seed, 480, 240, 2, baseline=(0.1, 0.3))
 
 
-def config1(seed: int = 0, W: int = 1920, H: int = 960, n_views: int = 6, workers: int = 8) -> Keyframe:
+def config1(seed: int = 0, W: int = 1920, H: int = 960, n_views: int = 6, workers: int = 8,
+            base_freq: float = TEX_FREQ_C1) -> Keyframe:
     """1920x960, reference + 6 neighbours, room + 20 spheres + tube, baselines 0.05-0.5 m."""
-    return make_keyframe(seed, W, H, n_views, baseline=(0.05, 0.5), spheres=20, tube=True, workers=workers)
+    return make_keyframe(seed, W, H, n_views, baseline=(0.05, 0.5), spheres=20, tube=True, workers=workers,
+                         base_freq=base_freq)
 
 
 def sequence_gpu(seed: int, n: int, W: int, H: int, spacing: float = 0.2, mask_frac: float = 0.15,
-                 with_gt: bool = True):
+                 with_gt: bool = True, base_freq: float = TEX_FREQ_C1):
     """n keyframes along the seeded trajectory through the config-1 scene
     (room + 20 spheres + tube, spheres kept clear of the path), rendered at
     W x H on the GPU: dict with poses (n, 12), images (n x (H, W) uint8), gt
     (n x (H, W) float32 depth, or None entries without with_gt) and the mask."""
     s_scene, s_photo = _split_seeds(seed ^ 0x5E0, 2)
-    scene = make_scene(s_scene, spheres=20, tube=True, cam_region=2.7)
+    scene = make_scene(s_scene, spheres=20, tube=True, cam_region=2.7, base_freq=base_freq)
     poses = trajectory(seed, n, spacing)
     prng = np.random.default_rng(s_photo)
     images, gts = [], []

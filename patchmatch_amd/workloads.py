@@ -163,7 +163,8 @@ def run_batched(stream: Stream, params, n: int, seed: int):
 
 
 def run_pipeline(seed: int = 0, n: int = 30, W: int = 1920, H: int = 960, views=(4, 8), params=None,
-                 thr: float = 0.01, max_reproj_px: float = 1.0, max_normal_deg: float = 10.0, keep: bool = False):
+                 thr: float = 0.01, max_reproj_px: float = 1.0, max_normal_deg: float = 10.0, keep: bool = False,
+                 base_freq: float | None = None):
     """The whole dense path over n keyframes of one trajectory through the
     config-1 scene (0.2 m spacing), as the reference's two threads run it:
       densify  every keyframe against its U{views} nearest trajectory
@@ -176,12 +177,14 @@ def run_pipeline(seed: int = 0, n: int = 30, W: int = 1920, H: int = 960, views=
     The same windows are also cleaned with pmp_clean_ex's round trip within
     max_reproj_px, normals within max_normal_deg, and both.  Accuracy
     against the rendered ground truth after each stage; with keep, the maps
-    of every stage are returned too (for parity checks)."""
+    of every stage are returned too (for parity checks).  base_freq: the
+    texture scale (default: the config-1 scene's)."""
     from . import synth
     from ._lib import Context, Keyframe
     from .params import config1_params
     p = params if params is not None else config1_params(seed=seed)
-    seq = synth.sequence_gpu(seed, n, W, H, 0.2)
+    seq = synth.sequence_gpu(seed, n, W, H, 0.2,
+                             base_freq=synth.TEX_FREQ_C1 if base_freq is None else base_freq)
     poses, imgs, gts, mask = seq["poses"], seq["images"], seq["gt"], seq["mask"]
     rng = np.random.default_rng(seed + 0x2C)
     ctx = Context(0)

@@ -31,9 +31,10 @@ def kf_c0():
 
 @pytest.fixture(scope="session")
 def kf_small6():
-    """A reduced config-1 scene (spheres + tube, 6 neighbours) at 320x160."""
+    """A reduced config-1 scene (spheres + tube, 6 neighbours) at 320x160, with
+    config 0's texture scale (the 1920-pixel scale would alias at 320 pixels)."""
     from patchmatch_amd import synth
-    return cached("s6", lambda: synth.config1(3, W=320, H=160, n_views=6))
+    return cached("s6", lambda: synth.config1(3, W=320, H=160, n_views=6, base_freq=synth.TEX_FREQ_C0))
 
 
 @pytest.fixture(scope="session")

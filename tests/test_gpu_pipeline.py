@@ -1,7 +1,8 @@
 """The whole dense path over one trajectory (densify -> reference Cleaner
 -> Pruner, plus the Cleaner with the round-trip and normal tests), as
 workloads.run_pipeline runs it at 1920x960 for the 30-keyframe sequence:
-here 14 keyframes at 480x240.
+here 14 keyframes at 480x240 (with config 0's texture scale, which suits
+that width).
 
 Every stage is checked against the oracle on the stage's own inputs (the
 first cleaning window with and without the tests, the first prune);
@@ -11,7 +12,7 @@ import numpy as np
 import pytest
 
 from oracle import or_py
-from patchmatch_amd import workloads
+from patchmatch_amd import synth, workloads
 from patchmatch_amd.params import config1_params
 
 pytestmark = pytest.mark.gpu
@@ -24,7 +25,7 @@ def _bits(a):
 def test_pipeline_stages_match_oracle():
     n, W, H = 14, 480, 240
     p = config1_params(seed=3, iterations=4)
-    r = workloads.run_pipeline(seed=3, n=n, W=W, H=H, params=p, keep=True)
+    r = workloads.run_pipeline(seed=3, n=n, W=W, H=H, params=p, keep=True, base_freq=synth.TEX_FREQ_C0)
     acc = r["accuracy"]
     print(f"[pipeline] {acc}", flush=True)
     maps, poses, imgs = r["_maps"], r["_poses"], r["_images"]
