@@ -50,14 +50,14 @@ HBM_PEAK_GBS = 8000.0
 # L2-to-memory bytes per k_propagate_q launch on the 1920x960 + 6-view
 # workload, from rocprofv3 FETCH_SIZE and WRITE_SIZE scaled by a copy of known
 # size (profiles/k_propagate_q_c1_1920x960_6views.md); None for other workloads.
-TRAFFIC_BYTES_PER_LAUNCH_C1 = int((186_371 * 2.00 + 42_982 * 1.00) * 1024)
+TRAFFIC_BYTES_PER_LAUNCH_C1 = int((181_613 * 2.00 + 43_038 * 1.00) * 1024)
 TRAFFIC_SOURCE = ("profile-derived, not measured in this run: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE of "
                   "k_propagate_q on this workload, scaled by a streaming copy of known size "
                   "(profiles/k_propagate_q_c1_1920x960_6views.md)")
 # VALU wave-instructions per k_propagate_q launch on the 1920x960 + 6-view
 # workload (rocprofv3 --pmc SQ_INSTS_VALU, same profile): the issue side of
 # the roofline, against one wave64 VALU instruction per CU per clock.
-VALU_WAVE_INSTS_PER_LAUNCH_C1 = 2.509e9
+VALU_WAVE_INSTS_PER_LAUNCH_C1 = 2.539e9
 CUS, CLOCK_HZ = 256, 2.4e9
 
 

@@ -277,7 +277,7 @@ typedef struct pmp_keyframe {
  * gives bit-identical maps, only the speed differs. */
 typedef struct pmp_batch_config {
     int32_t tile_rows;        /* rows per propagation tile (2, 4 or 8); the block has 64*tile_rows threads */
-    int32_t blocks_per_cu;    /* persistent propagation blocks per CU (>= 1) */
+    int32_t blocks_per_cu;    /* persistent propagation blocks per CU (0: as many as fit) */
     int32_t use_graph;        /* 1: replay the whole batch (init, 2*iterations passes, finalize) as one hipGraph */
     int32_t xcd_bands;        /* 1: each XCD drains its own contiguous band of tiles first */
 } pmp_batch_config;
@@ -322,7 +322,8 @@ typedef struct pmp_server_opts {
                                    admitted (oldest first) only while the keyframes in flight plus
                                    it stay within the budget, or when nothing is in flight; 0: no
                                    budget (only max_slots limits) */
-    pmp_batch_config config;    /* tile_rows, blocks_per_cu, xcd_bands of the step kernels */
+    pmp_batch_config config;    /* tile_rows, blocks_per_cu, xcd_bands of the step kernels;
+                                   tile_rows 0: the default (4, i.e. 8 x 64 tiles, 3 blocks per CU) */
     int32_t tail_stages;        /* latency-first admission: while at most one request is queued, it
                                    joins the requests in flight only once each of them has at most
                                    this many stages left (a request's stages: init, 2 per

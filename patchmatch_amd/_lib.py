@@ -605,8 +605,9 @@ class Context:
 class Server:
     """pmp_server: densification requests served with continuous batching."""
 
-    def __init__(self, device=0, max_slots=16, hbm_budget=0, edge_mode=0, tile_rows=8, blocks_per_cu=0,
+    def __init__(self, device=0, max_slots=16, hbm_budget=0, edge_mode=0, tile_rows=0, blocks_per_cu=0,
                  xcd_bands=1, inflight_kpx=0, tail_stages=0):
+        """tile_rows 0: the library's default launch shape."""
         self.L = load()
         o = ServerOpts(device, max_slots, hbm_budget, edge_mode, inflight_kpx,
                        BatchConfig(tile_rows, blocks_per_cu, 1, xcd_bands), tail_stages)

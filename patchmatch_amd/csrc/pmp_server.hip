@@ -32,7 +32,7 @@ class GpuExec final : public sched::Executor {
     DescSet desc[2];
     hipEvent_t step_done[2] = {nullptr, nullptr};
     std::vector<SlotBuf> slots;
-    pmp_batch_config cfg{8, 0, 1, 1};
+    pmp_batch_config cfg = kDefaultConfig;
     int edge_mode = PMP_EDGE_WRAP;
     // ctx->images and ctx->last_error: touched by the submitting thread
     // (upload, pack) and the scheduler thread (drop, slot_grow, step_issue).
@@ -286,7 +286,7 @@ int pmp_server_create(pmp_server** out, const pmp_server_opts* opts) {
     if (opts) o = *opts;
     if (o.max_slots == 0) o.max_slots = 16;
     if (o.hbm_budget == 0) o.hbm_budget = 64ll << 30;
-    if (o.config.tile_rows == 0) o.config = pmp_batch_config{8, 0, 1, 1};
+    if (o.config.tile_rows == 0) o.config = kDefaultConfig;
     if (o.max_slots < 1 || o.max_slots > kMaxSlots || o.hbm_budget < 0 || o.edge_mode < 0 || o.edge_mode > 2 ||
         o.inflight_kpx < 0 || o.tail_stages < 0 ||
         (o.config.tile_rows != 2 && o.config.tile_rows != 4 && o.config.tile_rows != 8))
