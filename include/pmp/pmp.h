@@ -293,10 +293,11 @@ int pmp_batch_densify(pmp_ctx* ctx, const pmp_params* params, int32_t n, const p
 int pmp_batch_get_maps(pmp_ctx* ctx, int32_t k, float* plane4, float* depth, float* score, int32_t* nghbr);
 int pmp_batch_set_config(pmp_ctx* ctx, const pmp_batch_config* cfg);
 int pmp_batch_get_config(pmp_ctx* ctx, pmp_batch_config* cfg);
-/* Times one red-black pass of the batch under every configuration in the
- * search space (tile rows x blocks per CU), keeps the fastest as the context's
- * configuration and writes it to *best (nullable).  The batch's results are
- * not kept (run pmp_batch_densify afterwards). */
+/* Times the red-black passes of the batch (init, then every pass, from the
+ * same starting maps) under every configuration in the search space (tile
+ * rows x blocks per CU x XCD bands, at least 8 waves per CU), keeps the
+ * fastest as the context's configuration and writes it to *best (nullable).
+ * The batch's results are not kept (run pmp_batch_densify afterwards). */
 int pmp_batch_autotune(pmp_ctx* ctx, const pmp_params* params, int32_t n, const pmp_keyframe* kfs,
                        pmp_batch_config* best);
 

@@ -638,11 +638,13 @@ int pmp_batch_autotune(pmp_ctx* ctx, const pmp_params* params, int32_t n, const 
                 c.blocks_per_cu = bpc;
                 c.xcd_bands = bands;
                 if ((rc = queue_run(ctx, *q, c, zero, false, r))) break;
-                // same starting maps for every candidate: init, then time the
-                // first passes (up to 3 iterations).  The first pass scores
-                // random planes whose gathers scatter; later passes are
-                // instruction-bound, so both kinds weigh in the choice.
-                const int timed = std::min(2 * params->iterations, 6);
+                // same starting maps for every candidate: init, then time every
+                // pass.  The first passes score random planes (most candidates
+                // survive pre-rejection, the gathers scatter); the converged
+                // ones pre-reject most candidates and are instruction-bound.
+                // The shapes rank differently on the two kinds, so a prefix of
+                // passes misranks them.
+                const int timed = 2 * params->iterations;
                 if ((rc = q_clear(ctx, ctx->stream, q->batch, timed))) break;   // every timed pass's heads
                 if ((rc = q_init(ctx, ctx->stream, q->batch, r))) break;
                 hipError_t e = hipEventRecord(e0, ctx->stream);
