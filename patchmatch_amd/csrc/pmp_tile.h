@@ -52,11 +52,12 @@ __device__ __forceinline__ void init_pixel(const KfArgs& a, const float* sWc, in
 }
 
 // ---- red-black pass
-// A tile is R rows x 64 same-colour pixels (128 columns); thread (w, l) of the
-// 64R-thread block updates row i = R by + w, column j = 2 (64 bx + l) + ((i +
-// colour) & 1), i.e. every pixel of colour (i+j)&1 == colour once.  Candidates
-// are at odd distance (1 or 3), so they have the other colour and are not
-// written by this pass.
+// A tile is 2R rows x 32 same-colour pixels (64 columns); thread t of the
+// 64R-thread block updates row i = 2R by + t / 32, column
+// j = 2 (32 bx + t % 32) + ((i + colour) & 1), i.e. every pixel of colour
+// (i+j)&1 == colour once (each wave covers two rows).  Candidates are at odd
+// distance (1 or 3), so they have the other colour and are not written by
+// this pass.
 //
 // Propagation candidates are scored by whichever threads of the block are
 // free, in packed sweeps, and their raw scores handed back to the owning
