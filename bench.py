@@ -125,6 +125,46 @@ def gpu_c0_end_to_end(ctx, repeats: int = 5):
     return float(np.median(ts)) * 1e3
 
 
+def reference_defaults_keyframe():
+    """The reference's default operating point (example/patchmatch/config.yaml:39-44,
+    depthmap_computation_module.cc:211): a 640x320 depth map against 2
+    neighbours, 36-tap patch; box-room scene as in config 0."""
+    from patchmatch_amd import synth
+    from patchmatch_amd.params import params_from_json
+    kf = synth.make_keyframe(7, 640, 320, 2, baseline=(0.1, 0.3))
+    return kf, params_from_json({"base": "default", "iterations": 4})
+
+
+def reference_defaults_times(ctx, threads: int, repeats: int = 5):
+    """Median end-to-end densification time of that keyframe with pmp_default_params
+    (the reference's constants, pooled view fold) and 4 iterations, on the GPU
+    (images resident) and in the oracle on `threads` host threads."""
+    from oracle import or_py
+    kf, p = reference_defaults_keyframe()
+    ctx.load_keyframe(kf.ref, kf.views, kf.ref_pose, kf.view_poses, base_id=3000)
+    ctx.image_put(3999, kf.mask)
+    ctx.set_mask(3999)
+    ctx.densify(p)
+    ctx.synchronize()
+    g = []
+    for _ in range(repeats):
+        t = time.perf_counter()
+        ctx.densify(p)
+        ctx.synchronize()
+        g.append(time.perf_counter() - t)
+    or_py.set_threads(threads)
+    ok = or_py.Keyframe.from_synth(kf)
+    c = []
+    for _ in range(3):
+        t = time.perf_counter()
+        or_py.densify(ok, p, kf.mask)
+        c.append(time.perf_counter() - t)
+    return {"workload": "640x320 + 2 neighbours, 11x11 patch at stride 2, pooled view fold, 4 iterations "
+                        "(the reference's example configuration), box-room scene",
+            "gpu_ms": round(float(np.median(g)) * 1e3, 3), "oracle_s": round(float(np.median(c)), 3),
+            "oracle_threads": threads}
+
+
 def cpu_model() -> str:
     try:
         with open("/proc/cpuinfo") as f:
@@ -312,7 +352,8 @@ def main(argv=None):
             "cpu_model": cpu_model(), "nproc": os.cpu_count(),
             "c0_end_to_end_s": round(cpu_c0_end_to_end(threads), 3),
             "c0_gpu_end_to_end_ms": round(gpu_c0_end_to_end(ctx), 3),
-            "c0_note": "480x240 + 2 views, 7x7 patch, 4 iterations: median of 5 densifications, oracle and GPU"}
+            "c0_note": "480x240 + 2 views, 7x7 patch, 4 iterations: median of 5 densifications, oracle and GPU",
+            "reference_defaults": reference_defaults_times(ctx, threads)}
     ctx.close()
 
     if args.batched > 0 or args.served_requests > 0 or args.pipeline > 0:
