@@ -29,7 +29,9 @@ forms of the metric (patchmatch_amd/workloads.py):
   "pipeline": 30 keyframes of one trajectory at the plain size, densified
              against their 4-8 nearest neighbours, cleaned (5-map window,
              depth within 1 %, round trip within 1 px, normals within 10
-             degrees) and fused (Pruner): accuracy after each stage.
+             degrees) and fused (Pruner): accuracy after each stage; then
+             densified once more one by one with the densify thread's warm
+             start (the previous keyframe's result warped in).
 """
 from __future__ import annotations
 

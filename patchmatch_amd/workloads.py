@@ -175,10 +175,14 @@ def run_pipeline(seed: int = 0, n: int = 30, W: int = 1920, H: int = 960, views=
       fuse     the cleaned maps in order (DepthmapPruner: 5-deep queue, prune
                the head, pop it; dense_pointcloud_module.cc:56-182).
     The same windows are also cleaned with pmp_clean_ex's round trip within
-    max_reproj_px, normals within max_normal_deg, and both.  Accuracy
-    against the rendered ground truth after each stage; with keep, the maps
-    of every stage are returned too (for parity checks).  base_freq: the
-    texture scale (default: the config-1 scene's)."""
+    max_reproj_px, normals within max_normal_deg, and both.  Then the
+    keyframes are densified once more one by one in trajectory order with the
+    densify thread's warm start (ComputePatchMatch with PreInitResultWarp of
+    the previous keyframe's memorised result, depthmap.cc:434-516, 808-829):
+    random init, warp of the previous result, the red-black passes, copy-back,
+    memorise.  Accuracy against the rendered ground truth after each stage;
+    with keep, the maps of every stage are returned too (for parity checks).
+    base_freq: the texture scale (default: the config-1 scene's)."""
     from . import synth
     from ._lib import Context, Keyframe
     from .params import config1_params
@@ -227,6 +231,22 @@ def run_pipeline(seed: int = 0, n: int = 30, W: int = 1920, H: int = 960, views=
         checked[name] = {c: ctx.clean_ex([maps[m][1] for m in windows[c]], [maps[m][0] for m in windows[c]],
                                          [poses[m] for m in windows[c]], thr, 3, px, deg) for c in centres}
     t5 = time.perf_counter()
+    # warm start: each keyframe pre-initialised from the previous one's result
+    from ._lib import PMP_MAPS_FINAL
+    ctx.forget_result()
+    warm, warped = {}, []
+    for k in range(n):
+        d = descs[k]
+        nb = [int(d.view_ids[v]) for v in range(d.n_views)]
+        ctx.set_views(k + 1, poses[k], nb, [poses[m - 1] for m in nb])
+        ctx.random_initialization(p)
+        if k > 0:
+            warped.append(ctx.preinit_warp(p, mask))
+        ctx.red_black_pass(p, 0, p.iterations)
+        ctx.finalize(p, mask)
+        ctx.memorize_result()
+        warm[k] = ctx.get_maps(PMP_MAPS_FINAL)[1]
+    t6 = time.perf_counter()
     ctx.close()
 
     unm = mask > 0
@@ -255,7 +275,10 @@ def run_pipeline(seed: int = 0, n: int = 30, W: int = 1920, H: int = 960, views=
             gt = gts[h][kps[:, 0], kps[:, 1]]
             prel.append(np.abs(dist - gt) / gt)
     prel = np.concatenate(prel) if prel else np.zeros(0)
-    acc = {"densified": summary({c: maps[c][1] for c in centres}), "cleaned": summary(cleaned)}
+    acc = {"densified": summary({c: maps[c][1] for c in centres}), "cleaned": summary(cleaned),
+           "densified_warm_start": summary({c: warm[c] for c in centres})}
+    acc["densified_warm_start"]["warped_candidates_accepted_mean"] = round(float(np.mean(warped)), 1) if warped \
+        else 0.0
     acc.update({f"cleaned_{k}": summary(v) for k, v in checked.items()})
     acc["fused_points"] = {"count": int(npts), "prunes": len(fused),
                            "absrel_le_1pct": round(float(np.mean(prel <= 0.01)), 4) if prel.size else None,
@@ -264,13 +287,14 @@ def run_pipeline(seed: int = 0, n: int = 30, W: int = 1920, H: int = 960, views=
         "keyframes": n, "width": W, "height": H, "views": list(views), "same_depth_thr": thr,
         "max_reproj_px": max_reproj_px, "max_normal_deg": max_normal_deg,
         "seconds": {"upload": round(t1 - t0, 3), "densify_batched": round(t2 - t1, 3), "clean": round(t3 - t2, 3),
-                    "fuse": round(t4 - t3, 3), "clean_with_checks_x3": round(t5 - t4, 3)},
+                    "fuse": round(t4 - t3, 3), "clean_with_checks_x3": round(t5 - t4, 3),
+                    "densify_warm_start_one_by_one": round(t6 - t5, 3)},
         "keyframes_per_s": round(n / (t4 - t1), 3),
         "accuracy": acc,
     }
     if keep:
         out.update(_maps=maps, _cleaned=cleaned, _checked=checked, _windows=windows, _poses=poses, _images=imgs,
-                   _fused=fused, _pruned=pruned_maps)
+                   _fused=fused, _pruned=pruned_maps, _warm=warm)
     return out
 
 
