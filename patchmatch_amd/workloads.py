@@ -176,11 +176,13 @@ def run_pipeline(seed: int = 0, n: int = 30, W: int = 1920, H: int = 960, views=
                the head, pop it; dense_pointcloud_module.cc:56-182).
     The same windows are also cleaned with pmp_clean_ex's round trip within
     max_reproj_px, normals within max_normal_deg, and both.  Then the
-    keyframes are densified once more one by one in trajectory order with the
+    keyframes are densified again one by one in trajectory order with the
     densify thread's warm start (ComputePatchMatch with PreInitResultWarp of
     the previous keyframe's memorised result, depthmap.cc:434-516, 808-829):
     random init, warp of the previous result, the red-black passes, copy-back,
-    memorise.  Accuracy against the rendered ground truth after each stage;
+    memorise; at the params' iteration count and at 4 and 2 iterations (the
+    reference's example and code defaults, config.cc:137-190), each with the
+    keyframes' latency (all but the first, which has nothing to warp).  Accuracy against the rendered ground truth after each stage;
     with keep, the maps of every stage are returned too (for parity checks).
     base_freq: the texture scale (default: the config-1 scene's)."""
     from . import synth
@@ -231,21 +233,34 @@ def run_pipeline(seed: int = 0, n: int = 30, W: int = 1920, H: int = 960, views=
         checked[name] = {c: ctx.clean_ex([maps[m][1] for m in windows[c]], [maps[m][0] for m in windows[c]],
                                          [poses[m] for m in windows[c]], thr, 3, px, deg) for c in centres}
     t5 = time.perf_counter()
-    # warm start: each keyframe pre-initialised from the previous one's result
+    # warm start: each keyframe pre-initialised from the previous one's result,
+    # at the params' iteration count and at the reference's defaults of 4
+    # (example config) and 2 (code default); latency per keyframe from the
+    # views' selection to the final maps on the device
     from ._lib import PMP_MAPS_FINAL
-    ctx.forget_result()
-    warm, warped = {}, []
-    for k in range(n):
-        d = descs[k]
-        nb = [int(d.view_ids[v]) for v in range(d.n_views)]
-        ctx.set_views(k + 1, poses[k], nb, [poses[m - 1] for m in nb])
-        ctx.random_initialization(p)
-        if k > 0:
-            warped.append(ctx.preinit_warp(p, mask))
-        ctx.red_black_pass(p, 0, p.iterations)
-        ctx.finalize(p, mask)
-        ctx.memorize_result()
-        warm[k] = ctx.get_maps(PMP_MAPS_FINAL)[1]
+
+    def warm_run(iterations):
+        ctx.forget_result()
+        maps_w, warped, lat = {}, [], []
+        for k in range(n):
+            d = descs[k]
+            nb = [int(d.view_ids[v]) for v in range(d.n_views)]
+            ts = time.perf_counter()
+            ctx.set_views(k + 1, poses[k], nb, [poses[m - 1] for m in nb])
+            ctx.random_initialization(p)
+            if k > 0:
+                warped.append(ctx.preinit_warp(p, mask))
+            ctx.red_black_pass(p, 0, iterations)
+            ctx.finalize(p, mask)
+            ctx.memorize_result()
+            ctx.synchronize()
+            lat.append((time.perf_counter() - ts) * 1e3)
+            maps_w[k] = ctx.get_maps(PMP_MAPS_FINAL)[1]
+        return maps_w, warped, lat
+
+    warm_iters = sorted({p.iterations, 4, 2}, reverse=True)
+    warm_runs = {it: warm_run(it) for it in warm_iters}
+    warm = warm_runs[p.iterations][0]
     t6 = time.perf_counter()
     ctx.close()
 
@@ -275,10 +290,13 @@ def run_pipeline(seed: int = 0, n: int = 30, W: int = 1920, H: int = 960, views=
             gt = gts[h][kps[:, 0], kps[:, 1]]
             prel.append(np.abs(dist - gt) / gt)
     prel = np.concatenate(prel) if prel else np.zeros(0)
-    acc = {"densified": summary({c: maps[c][1] for c in centres}), "cleaned": summary(cleaned),
-           "densified_warm_start": summary({c: warm[c] for c in centres})}
-    acc["densified_warm_start"]["warped_candidates_accepted_mean"] = round(float(np.mean(warped)), 1) if warped \
-        else 0.0
+    acc = {"densified": summary({c: maps[c][1] for c in centres}), "cleaned": summary(cleaned)}
+    for it, (mw, warped, lat) in warm_runs.items():
+        key = "densified_warm_start" if it == p.iterations else f"densified_warm_start_{it}_iterations"
+        acc[key] = summary({c: mw[c] for c in centres})
+        acc[key]["warped_candidates_accepted_mean"] = round(float(np.mean(warped)), 1) if warped else 0.0
+        acc[key]["latency_ms"] = {"p50": round(float(np.percentile(lat[1:], 50)), 3),
+                                  "p99": round(float(np.percentile(lat[1:], 99)), 3)}
     acc.update({f"cleaned_{k}": summary(v) for k, v in checked.items()})
     acc["fused_points"] = {"count": int(npts), "prunes": len(fused),
                            "absrel_le_1pct": round(float(np.mean(prel <= 0.01)), 4) if prel.size else None,
@@ -289,6 +307,7 @@ def run_pipeline(seed: int = 0, n: int = 30, W: int = 1920, H: int = 960, views=
         "seconds": {"upload": round(t1 - t0, 3), "densify_batched": round(t2 - t1, 3), "clean": round(t3 - t2, 3),
                     "fuse": round(t4 - t3, 3), "clean_with_checks_x3": round(t5 - t4, 3),
                     "densify_warm_start_one_by_one": round(t6 - t5, 3)},
+        "warm_start_iterations": warm_iters,
         "keyframes_per_s": round(n / (t4 - t1), 3),
         "accuracy": acc,
     }
