@@ -347,6 +347,15 @@ typedef struct pmp_request {
     const uint8_t* mask_pixels;
     int64_t n_landmarks;
     const double* landmark_xyz;
+    /* Warm start (PreInitResultWarp, depthmap.cc:434-516): the previous
+     * keyframe's final plane (4 floats per pixel) and depth maps, of this
+     * request's size, and that keyframe's pose; after the random
+     * initialisation (and the landmark seeding) each previous pixel's plane is
+     * warped into this keyframe and kept where it scores better.  All three
+     * null: no warm start.  Copied at submit. */
+    const float* prev_plane4;
+    const float* prev_depth;
+    const pmp_pose* prev_pose;
 } pmp_request;
 
 typedef struct pmp_request_info {

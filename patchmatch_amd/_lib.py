@@ -102,7 +102,8 @@ class ServerOpts(ctypes.Structure):
 class Request(ctypes.Structure):
     _fields_ = [("kf", Keyframe), ("width", ctypes.c_int32), ("height", ctypes.c_int32),
                 ("ref_pixels", _u8p), ("view_pixels", _u8p * PMP_MAX_VIEWS), ("mask_pixels", _u8p),
-                ("n_landmarks", ctypes.c_int64), ("landmark_xyz", _f64p)]
+                ("n_landmarks", ctypes.c_int64), ("landmark_xyz", _f64p),
+                ("prev_plane4", _f32p), ("prev_depth", _f32p), ("prev_pose", ctypes.POINTER(Pose))]
 
 
 class RequestInfo(ctypes.Structure):
@@ -641,9 +642,11 @@ class Server:
             raise PmpError(f"{what}: {STATUS.get(rc, rc)}: {msg}")
 
     def submit(self, p: Params, ref_id, ref_pose, view_ids, view_poses, shape, ref=None, views=None, mask_id=-1,
-               mask=None, min_depth=0.0, max_depth=0.0, landmarks=None) -> int:
+               mask=None, min_depth=0.0, max_depth=0.0, landmarks=None, prev=None) -> int:
         """Queue one keyframe; arrays are needed only for ids the pool does not hold.
-        landmarks: optional (n, 3) world positions that seed the maps after init."""
+        landmarks: optional (n, 3) world positions that seed the maps after init.
+        prev: optional warm start (plane (H, W, 4), depth (H, W), pose (12,)): the
+        previous keyframe's final maps and pose, warped in after init."""
         H, W = shape
         r = Request()
         r.kf = Keyframe.make(ref_id, ref_pose, view_ids, view_poses, mask_id, min_depth, max_depth)
@@ -667,6 +670,16 @@ class Server:
             keep.append(lm)
             r.n_landmarks = lm.shape[0]
             r.landmark_xyz = lm.ctypes.data_as(_f64p)
+        if prev is not None:
+            pl = np.ascontiguousarray(prev[0], dtype=np.float32)
+            dp = np.ascontiguousarray(prev[1], dtype=np.float32)
+            if pl.shape != (H, W, 4) or dp.shape != (H, W):
+                raise ValueError(f"previous maps {pl.shape}, {dp.shape} differ from the request's {(H, W)}")
+            pose = Pose.from12(prev[2])
+            keep += [pl, dp, pose]
+            r.prev_plane4 = pl.ctypes.data_as(_f32p)
+            r.prev_depth = dp.ctypes.data_as(_f32p)
+            r.prev_pose = ctypes.pointer(pose)
         t = ctypes.c_uint64()
         self._chk(self.L.pmp_server_submit(self.h, ctypes.byref(r), ctypes.byref(p), ctypes.byref(t)),
                   "pmp_server_submit")

@@ -325,6 +325,28 @@ int preinit_landmarks_slot(pmp_ctx* ctx, const SlotDesc* slot_dev, int W, int H,
     return run_stage(ctx, q, s, slot_dev, (size_t)W * H, pa, zero, false, nullptr);
 }
 
+int preinit_warp_slot(pmp_ctx* ctx, const SlotDesc* slot_dev, int W, int H, const double* pose,
+                      const double* prev_pose, const float* prev_plane4, const float* prev_depth, bool zero) {
+    const size_t P = (size_t)W * H;
+    QueueState* q = nullptr;
+    int rc = queue_state(ctx, q);
+    PreState* s = nullptr;
+    if (!rc) rc = pre_state(ctx, s);
+    if (!rc) rc = ensure_cands(ctx, s, P, P);
+    size_t c1 = s ? s->prev_cap : 0;
+    if (!rc) rc = grow(ctx, s->prev_plane, s->prev_cap, P);
+    if (!rc) rc = grow(ctx, s->prev_depth, c1, P);
+    if (rc) return rc;
+    PMP_HIP(ctx, hipMemcpyAsync(s->prev_plane, prev_plane4, P * sizeof(float4), hipMemcpyHostToDevice, ctx->stream));
+    PMP_HIP(ctx, hipMemcpyAsync(s->prev_depth, prev_depth, P * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+    s->prev_valid = false;   // the buffers no longer hold a memorised one-keyframe result
+    PreArgs pa;
+    memcpy(pa.pose, pose, sizeof(pa.pose));
+    memcpy(pa.prev_pose, prev_pose, sizeof(pa.prev_pose));
+    pa.n = (int)P;
+    return run_stage(ctx, q, s, slot_dev, P, pa, zero, true, nullptr);
+}
+
 }  // namespace pmp
 
 using namespace pmp;

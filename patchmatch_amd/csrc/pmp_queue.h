@@ -104,6 +104,13 @@ int queue_state(pmp_ctx* ctx, QueueState*& out);
 // Landmark seeding of one slot of an uploaded descriptor set (pmp_preinit.hip).
 int preinit_landmarks_slot(pmp_ctx* ctx, const SlotDesc* slot_dev, int W, int H, const double* pose, int64_t n,
                            const double* xyz, bool zero);
+// Warm start of one keyframe slot of a descriptor set (the served form): the
+// previous keyframe's final plane4 / depth maps (W x H pixels, pinned host
+// memory valid until the stream has passed this stage) and its pose, warped
+// into the slot's maps as pmp_preinit_warp does.  Stream-ordered on
+// ctx->stream, no host synchronisation (scratch growth aside).
+int preinit_warp_slot(pmp_ctx* ctx, const SlotDesc* slot_dev, int W, int H, const double* pose,
+                      const double* prev_pose, const float* prev_plane4, const float* prev_depth, bool zero);
 int desc_alloc(pmp_ctx* ctx, DescSet& d, int cap);
 void desc_free(DescSet& d);
 // Waits until the staging copy of d may be rewritten (its previous upload has executed).

@@ -29,6 +29,7 @@ Server::~Server() {
     for (auto& kv : reqs_) {
         give_host(kv.second->out);
         give_host(kv.second->lm);
+        give_host(kv.second->prev);
         delete kv.second;
     }
     for (auto& b : free_host_) ex_->host_free(b.p);
@@ -126,6 +127,9 @@ void Server::retire(Req* r) {
     unpin(r);
     give_host(r->lm);
     r->job.lm = nullptr;
+    give_host(r->prev);
+    r->job.prev_plane4 = nullptr;
+    r->job.prev_depth = nullptr;
     r->t_done = now_ms();
     if (r->rc == PMP_OK) {
         r->state = kDone;
@@ -388,6 +392,11 @@ int Server::submit(const pmp_request& req, const pmp_params& p, uint64_t* ticket
         err = "n_landmarks must be in [0, 2^31) with landmark_xyz given";
         return PMP_EINVAL;
     }
+    const bool warm = req.prev_plane4 || req.prev_depth || req.prev_pose;
+    if (warm && !(req.prev_plane4 && req.prev_depth && req.prev_pose)) {
+        err = "a warm start needs prev_plane4, prev_depth and prev_pose";
+        return PMP_EINVAL;
+    }
     if (ex_->check_params(p, err) != PMP_OK) return PMP_EINVAL;
     if (ex_->map_bytes((int64_t)W * H) > o_.budget) {
         err = "keyframe maps larger than the HBM budget";
@@ -404,6 +413,26 @@ int Server::submit(const pmp_request& req, const pmp_params& p, uint64_t* ticket
     for (int v = 0; v < kf.n_views && rc == PMP_OK; ++v)
         rc = acquire(kf.view_ids[v], req.view_pixels[v], W, H, true, r, err);
     if (rc == PMP_OK && kf.mask_id >= 0) rc = acquire((uint64_t)kf.mask_id, req.mask_pixels, W, H, false, r, err);
+    if (rc == PMP_OK && warm) {
+        // the previous result into pinned memory (copied without the lock:
+        // r is not visible to the scheduler before it is queued)
+        const size_t P = (size_t)W * H;
+        {
+            std::lock_guard<std::mutex> l2(mu_);
+            r->prev = take_host(20 * P);
+        }
+        if (!r->prev.p) {
+            err = "pinned host buffer of the previous result";
+            rc = PMP_ENOMEM;
+        } else {
+            memcpy(r->prev.p, req.prev_plane4, 16 * P);
+            memcpy(r->prev.p + 16 * P, req.prev_depth, 4 * P);
+            r->job.prev_plane4 = (const float*)r->prev.p;
+            r->job.prev_depth = (const float*)(r->prev.p + 16 * P);
+            memcpy(r->job.prev_pose, req.prev_pose->R, 9 * sizeof(double));
+            memcpy(r->job.prev_pose + 9, req.prev_pose->t, 3 * sizeof(double));
+        }
+    }
     std::lock_guard<std::mutex> lk(mu_);
     if (rc == PMP_OK && req.n_landmarks > 0) {
         const size_t bytes = 3 * (size_t)req.n_landmarks * sizeof(double);
@@ -419,6 +448,7 @@ int Server::submit(const pmp_request& req, const pmp_params& p, uint64_t* ticket
     }
     if (rc != PMP_OK) {
         give_host(r->lm);
+        give_host(r->prev);
         unpin(r);
         delete r;
         cv_room_.notify_all();
@@ -483,6 +513,7 @@ int Server::fetch(uint64_t ticket, float* plane4, float* depth, float* score, in
     }
     give_host(r->out);
     give_host(r->lm);
+    give_host(r->prev);
     reqs_.erase(it);
     delete r;
     return rc;

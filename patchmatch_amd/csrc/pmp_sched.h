@@ -43,7 +43,8 @@ namespace sched {
 
 // One request as the executor sees it.  out: host buffer of W*H*28 bytes
 // (plane 16 B | depth | score | nghbr per pixel, each map contiguous), written
-// by the step that finalizes the request; lm: n_lm landmark positions.
+// by the step that finalizes the request; lm: n_lm landmark positions;
+// prev_*: the warm start's previous result.
 struct Job {
     uint64_t ticket = 0;
     pmp_keyframe kf{};
@@ -52,6 +53,12 @@ struct Job {
     int last_stage = 1;
     const double* lm = nullptr;
     int64_t n_lm = 0;
+    // warm start: the previous keyframe's plane (4 floats per pixel) and depth
+    // maps of W x H pixels, in pinned host memory, and its pose (R row-major,
+    // then t); prev_plane4 null: none
+    const float* prev_plane4 = nullptr;
+    const float* prev_depth = nullptr;
+    double prev_pose[12] = {};
     uint8_t* out = nullptr;
 };
 
@@ -135,7 +142,7 @@ class Server {
         std::string error;
         double t_submit = 0.0, t_admit = 0.0, t_done = 0.0;
         std::vector<uint64_t> pins;
-        HostBuf out, lm;
+        HostBuf out, lm, prev;
     };
     struct Entry {
         int W = 0, H = 0;

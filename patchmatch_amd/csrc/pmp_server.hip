@@ -176,7 +176,7 @@ class GpuExec final : public sched::Executor {
         DescSet& d = desc[w];
         int items_init[kMaxSlots] = {0}, items_prop[kMaxSlots] = {0}, items_rows[kMaxSlots] = {0};
         bool post = false;
-        std::vector<const sched::StepItem*> fin, seeds;
+        std::vector<const sched::StepItem*> fin, seeds, warms;
         int rc = PMP_OK;
         // the descriptors read the pool, and errors are reported through the context
         std::lock_guard<std::mutex> lk(pool_mu);
@@ -203,6 +203,7 @@ class GpuExec final : public sched::Executor {
                 if (it.stage == 0) {
                     items_init[it.slot] = its[0];
                     if (j->n_lm > 0) seeds.push_back(&it);
+                    if (j->prev_plane4) warms.push_back(&it);
                 } else if (it.stage < j->last_stage) {
                     items_prop[it.slot] = its[1];
                 } else {
@@ -226,6 +227,14 @@ class GpuExec final : public sched::Executor {
             const sched::Job* j = it->job;
             rc = preinit_landmarks_slot(ctx, d.d_slots + it->slot, j->W, j->H, (const double*)&j->kf.ref_pose,
                                         j->n_lm, j->lm, edge_mode == PMP_EDGE_ZERO);
+        }
+        // then the warm starts (PreInitResultWarp after PreInitSparsePCL2,
+        // depthmap.cc:669-719)
+        for (const sched::StepItem* it : warms) {
+            if (rc) break;
+            const sched::Job* j = it->job;
+            rc = preinit_warp_slot(ctx, d.d_slots + it->slot, j->W, j->H, (const double*)&j->kf.ref_pose,
+                                   j->prev_pose, j->prev_plane4, j->prev_depth, edge_mode == PMP_EDGE_ZERO);
         }
         if (!rc && d.h_hdr->prop.n) rc = q_prop(ctx, ctx->stream, d, run, 0);
         if (!rc && d.h_hdr->rows.n) rc = q_finalize(ctx, ctx->stream, d, run);

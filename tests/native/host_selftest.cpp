@@ -60,6 +60,7 @@ struct SimExec final : sched::Executor {
     std::set<uint64_t> reading[2];           // images read by the step in flight in each set
     std::map<uint64_t, std::vector<int>> stages;   // ticket -> stages run, in order
     std::map<std::pair<uint64_t, int>, int> step_of;   // (ticket, stage) -> index of the step that issued it
+    std::map<uint64_t, std::vector<double>> prev_seen;  // ticket -> warm-start values read at stage 0
     std::atomic<int> issues{0}, waits{0};
     // fault injection
     uint64_t fail_upload_id = ~0ull, fail_pack_id = ~0ull;
@@ -175,6 +176,13 @@ struct SimExec final : sched::Executor {
             if (it.stage == 0 && j->n_lm > 0) {
                 volatile double sink = j->lm[3 * j->n_lm - 1];   // the landmarks must be readable
                 (void)sink;
+            }
+            if (it.stage == 0 && j->prev_plane4) {
+                // the previous result must be readable in full at the warm start
+                const size_t P = (size_t)j->W * j->H;
+                if (!j->prev_depth) bad("warm start without previous depth");
+                else prev_seen[j->ticket] = {j->prev_plane4[0], j->prev_plane4[4 * P - 1], j->prev_depth[0],
+                                             j->prev_depth[P - 1], j->prev_pose[0], j->prev_pose[11]};
             }
             if (it.stage == j->last_stage) {
                 const size_t P = (size_t)j->W * j->H;
@@ -563,6 +571,47 @@ static void test_destroy_finishes() {
     CHECK(ex.violations == 0);
 }
 
+// 5b. A warm start travels with its request as a copy made at submit (the
+// caller's maps may go right after), reaches the executor at stage 0, and is
+// all or nothing.
+static void test_warm_start() {
+    const int W = 16, H = 8;
+    const size_t P = (size_t)W * H;
+    SimExec ex(1ll << 30, 2);
+    uint64_t t0 = 0, t1 = 0;
+    {
+        sched::Options o;
+        o.max_slots = 2;
+        o.budget = 1ll << 30;
+        Images im;
+        sched::Server s(&ex, o);
+        std::string err;
+        std::vector<float> plane(4 * P), depth(P);
+        for (size_t k = 0; k < 4 * P; ++k) plane[k] = 0.25f * (float)k;
+        for (size_t k = 0; k < P; ++k) depth[k] = 1.0f + (float)k;
+        pmp_pose pose;
+        for (int k = 0; k < 9; ++k) pose.R[k] = k;
+        for (int k = 0; k < 3; ++k) pose.t[k] = 9 + k;
+        pmp_request r = request(im, 1, 8, 2, W, H);
+        r.prev_plane4 = plane.data();
+        r.prev_depth = depth.data();
+        CHECK(s.submit(r, params(2, 0), &t0, err) == PMP_EINVAL);   // no pose: all or nothing
+        r.prev_pose = &pose;
+        CHECK(s.submit(r, params(2, 0), &t0, err) == PMP_OK);
+        std::fill(plane.begin(), plane.end(), -1.0f);                  // the request holds its own copy
+        std::fill(depth.begin(), depth.end(), -1.0f);
+        pmp_request cold = request(im, 2, 8, 2, W, H);
+        CHECK(s.submit(cold, params(2, 1), &t1, err) == PMP_OK);
+        CHECK(s.wait(t0, -1, err) == PMP_OK && s.wait(t1, -1, err) == PMP_OK);
+    }
+    stages_complete(ex, t0, 2);
+    stages_complete(ex, t1, 2);
+    CHECK(ex.prev_seen.count(t0) == 1 && ex.prev_seen.count(t1) == 0);
+    const std::vector<double>& v = ex.prev_seen[t0];
+    CHECK(v[0] == 0.0 && v[1] == 0.25 * (4 * P - 1) && v[2] == 1.0 && v[3] == (double)P && v[4] == 0.0 && v[5] == 11.0);
+    CHECK(ex.violations == 0);
+}
+
 // 6. Slot choice: a request takes the smallest free slot whose maps already
 // hold it, else the largest free slot, so slot maps grow only when no free
 // slot is large enough.
@@ -711,6 +760,7 @@ int main(int argc, char** argv) {
     test_reset();
     test_faults();
     test_destroy_finishes();
+    test_warm_start();
     test_slot_best_fit();
     test_tail_admission();
     if (!sched_only) test_frontend_and_writers();

@@ -79,6 +79,8 @@ def test_batch_and_server_struct_layout(tmp_path):
               ("offsetof(pmp_request, ref_pixels)", _lib.Request.ref_pixels.offset),
               ("offsetof(pmp_request, mask_pixels)", _lib.Request.mask_pixels.offset),
               ("offsetof(pmp_request, landmark_xyz)", _lib.Request.landmark_xyz.offset),
+              ("offsetof(pmp_request, prev_plane4)", _lib.Request.prev_plane4.offset),
+              ("offsetof(pmp_request, prev_pose)", _lib.Request.prev_pose.offset),
               ("sizeof(pmp_request_info)", ctypes.sizeof(_lib.RequestInfo)),
               ("sizeof(pmp_server_info)", ctypes.sizeof(_lib.ServerInfo)),
               ("offsetof(pmp_server_info, in_flight)", _lib.ServerInfo.in_flight.offset)]

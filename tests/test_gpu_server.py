@@ -181,6 +181,51 @@ def test_server_landmark_seeding(gpu_ctx):
         _check(srv.fetch(t1), want_plain[1], "request without landmarks beside it")
 
 
+def test_server_warm_start(gpu_ctx):
+    """A request with the previous keyframe's result (warm start) equals the
+    one-keyframe sequence densify(previous) -> pmp_memorize_result ->
+    init -> pmp_preinit_warp -> passes -> finalize, in the mix with a request
+    without one; the warm start changes the result."""
+    kfs = _keyframes()
+    want_plain = _plain(gpu_ctx, kfs)
+    seq = synth.sequence_gpu(21, 4, 320, 160, with_gt=False, base_freq=synth.TEX_FREQ_C0)
+    imgs, poses, mask = seq["images"], seq["poses"], seq["mask"]
+    ids = [9000 + k for k in range(4)]
+    for k in range(4):
+        gpu_ctx.image_put(ids[k], imgs[k])
+    p = config1_params(iterations=2, seed=77)
+    nb0, nb1 = [1, 2, 3], [0, 2, 3]
+    gpu_ctx.set_views(ids[0], poses[0], [ids[m] for m in nb0], [poses[m] for m in nb0])
+    gpu_ctx.densify(p, mask)
+    prev = gpu_ctx.get_maps(PMP_MAPS_FINAL)
+    gpu_ctx.memorize_result()
+    gpu_ctx.set_views(ids[1], poses[1], [ids[m] for m in nb1], [poses[m] for m in nb1])
+    gpu_ctx.random_initialization(p)
+    acc = gpu_ctx.preinit_warp(p, mask)
+    gpu_ctx.red_black_pass(p, 0, p.iterations)
+    gpu_ctx.finalize(p, mask)
+    want = gpu_ctx.get_maps(PMP_MAPS_FINAL)
+    gpu_ctx.forget_result()
+    gpu_ctx.densify(p, mask)
+    cold = gpu_ctx.get_maps(PMP_MAPS_FINAL)
+    print(f"[server warm start] {acc} warped candidates accepted on the plain path", flush=True)
+    assert acc > 1000
+    assert not np.array_equal(want[1].view(np.uint32), cold[1].view(np.uint32))
+    with Server(0, max_slots=4) as srv:
+        srv.pause(True)
+        t0 = srv.submit(p, ids[1], poses[1], [ids[m] for m in nb1], [poses[m] for m in nb1], mask.shape,
+                        ref=imgs[1], views=[imgs[m] for m in nb1], mask_id=9999, mask=mask,
+                        prev=(prev[0], prev[1], poses[0]))
+        t1 = _submit(srv, kfs, 1)
+        srv.pause(False)
+        assert srv.wait(t0, 60000) and srv.wait(t1, 60000)
+        _check(srv.fetch(t0), want, "warm-started request")
+        _check(srv.fetch(t1), want_plain[1], "request without a warm start beside it")
+        with pytest.raises(ValueError):
+            srv.submit(p, ids[1], poses[1], [ids[m] for m in nb1], [poses[m] for m in nb1], mask.shape,
+                       mask_id=9999, prev=(prev[0][:-1], prev[1], poses[0]))
+
+
 def test_server_staggered_arrivals(gpu_ctx):
     """Requests submitted from another thread while earlier ones run."""
     kfs = _keyframes()
