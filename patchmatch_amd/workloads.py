@@ -164,7 +164,7 @@ def run_batched(stream: Stream, params, n: int, seed: int):
 
 def run_pipeline(seed: int = 0, n: int = 30, W: int = 1920, H: int = 960, views=(4, 8), params=None,
                  thr: float = 0.01, max_reproj_px: float = 1.0, max_normal_deg: float = 10.0, keep: bool = False,
-                 base_freq: float | None = None):
+                 base_freq: float | None = None, warm_iterations=(4, 2), warm_views: int | None = None):
     """The whole dense path over n keyframes of one trajectory through the
     config-1 scene (0.2 m spacing), as the reference's two threads run it:
       densify  every keyframe against its U{views} nearest trajectory
@@ -180,11 +180,14 @@ def run_pipeline(seed: int = 0, n: int = 30, W: int = 1920, H: int = 960, views=
     densify thread's warm start (ComputePatchMatch with PreInitResultWarp of
     the previous keyframe's memorised result, depthmap.cc:434-516, 808-829):
     random init, warp of the previous result, the red-black passes, copy-back,
-    memorise; at the params' iteration count and at 4 and 2 iterations (the
-    reference's example and code defaults, config.cc:137-190), each with the
+    memorise; at the params' iteration count and at each of warm_iterations
+    (default 4 and 2: the reference's example and code defaults,
+    config.cc:137-190), each with the
     keyframes' latency (all but the first, which has nothing to warp).  Accuracy against the rendered ground truth after each stage;
     with keep, the maps of every stage are returned too (for parity checks).
-    base_freq: the texture scale (default: the config-1 scene's)."""
+    base_freq: the texture scale (default: the config-1 scene's).
+    warm_views: when given, the warm-start runs densify every keyframe against
+    exactly this many nearest trajectory neighbours instead of U{views}."""
     from . import synth
     from ._lib import Context, Keyframe
     from .params import config1_params
@@ -243,8 +246,10 @@ def run_pipeline(seed: int = 0, n: int = 30, W: int = 1920, H: int = 960, views=
         ctx.forget_result()
         maps_w, warped, lat = {}, [], []
         for k in range(n):
-            d = descs[k]
-            nb = [int(d.view_ids[v]) for v in range(d.n_views)]
+            if warm_views:
+                nb = [m + 1 for m in sorted((m for m in range(n) if m != k), key=lambda m: (abs(m - k), m))[:warm_views]]
+            else:
+                nb = [int(descs[k].view_ids[v]) for v in range(descs[k].n_views)]
             ts = time.perf_counter()
             ctx.set_views(k + 1, poses[k], nb, [poses[m - 1] for m in nb])
             ctx.random_initialization(p)
@@ -258,7 +263,7 @@ def run_pipeline(seed: int = 0, n: int = 30, W: int = 1920, H: int = 960, views=
             maps_w[k] = ctx.get_maps(PMP_MAPS_FINAL)[1]
         return maps_w, warped, lat
 
-    warm_iters = sorted({p.iterations, 4, 2}, reverse=True)
+    warm_iters = sorted({p.iterations, *warm_iterations}, reverse=True)
     warm_runs = {it: warm_run(it) for it in warm_iters}
     warm = warm_runs[p.iterations][0]
     t6 = time.perf_counter()
@@ -307,7 +312,7 @@ def run_pipeline(seed: int = 0, n: int = 30, W: int = 1920, H: int = 960, views=
         "seconds": {"upload": round(t1 - t0, 3), "densify_batched": round(t2 - t1, 3), "clean": round(t3 - t2, 3),
                     "fuse": round(t4 - t3, 3), "clean_with_checks_x3": round(t5 - t4, 3),
                     "densify_warm_start_one_by_one": round(t6 - t5, 3)},
-        "warm_start_iterations": warm_iters,
+        "warm_start_iterations": warm_iters, "warm_start_views": warm_views or list(views),
         "keyframes_per_s": round(n / (t4 - t1), 3),
         "accuracy": acc,
     }
