@@ -253,13 +253,25 @@ static int grow(pmp_ctx* ctx, T*& p, size_t& cap, size_t n) {
     return PMP_OK;
 }
 
+// Buffers that share one capacity grow together; after a failed allocation
+// the shared capacity is 0, so the next call reallocates all of them.
 static int ensure_cands(pmp_ctx* ctx, PreState* s, size_t n, size_t pixels) {
-    size_t c1 = s->cap, c2 = s->cap, c3 = s->cap;
-    int rc = grow(ctx, s->cpix, s->cap, n);
+    size_t c0 = s->cap, c1 = s->cap, c2 = s->cap, c3 = s->cap;
+    int rc = grow(ctx, s->cpix, c0, n);
     if (!rc) rc = grow(ctx, s->cplane, c1, n);
     if (!rc) rc = grow(ctx, s->cdep, c2, n);
     if (!rc) rc = grow(ctx, s->cscore, c3, n);
+    s->cap = rc ? 0 : c0;
     if (!rc) rc = grow(ctx, s->keys, s->keys_cap, pixels);
+    return rc;
+}
+
+static int ensure_prev(pmp_ctx* ctx, PreState* s, size_t P) {
+    size_t c0 = s->prev_cap, c1 = s->prev_cap;
+    int rc = grow(ctx, s->prev_plane, c0, P);
+    if (!rc) rc = grow(ctx, s->prev_depth, c1, P);
+    s->prev_cap = rc ? 0 : c0;
+    if (rc) s->prev_valid = false;
     return rc;
 }
 
@@ -333,9 +345,7 @@ int preinit_warp_slot(pmp_ctx* ctx, const SlotDesc* slot_dev, int W, int H, cons
     PreState* s = nullptr;
     if (!rc) rc = pre_state(ctx, s);
     if (!rc) rc = ensure_cands(ctx, s, P, P);
-    size_t c1 = s ? s->prev_cap : 0;
-    if (!rc) rc = grow(ctx, s->prev_plane, s->prev_cap, P);
-    if (!rc) rc = grow(ctx, s->prev_depth, c1, P);
+    if (!rc) rc = ensure_prev(ctx, s, P);
     if (rc) return rc;
     PMP_HIP(ctx, hipMemcpyAsync(s->prev_plane, prev_plane4, P * sizeof(float4), hipMemcpyHostToDevice, ctx->stream));
     PMP_HIP(ctx, hipMemcpyAsync(s->prev_depth, prev_depth, P * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
@@ -388,9 +398,7 @@ int pmp_memorize_result(pmp_ctx* ctx) {
     PreState* s = nullptr;
     int rc = pre_state(ctx, s);
     const size_t P = (size_t)ctx->W * ctx->H;
-    size_t c1 = s ? s->prev_cap : 0;
-    if (!rc) rc = grow(ctx, s->prev_plane, s->prev_cap, P);
-    if (!rc) rc = grow(ctx, s->prev_depth, c1, P);
+    if (!rc) rc = ensure_prev(ctx, s, P);
     if (rc) return rc;
     PMP_HIP(ctx, hipMemcpyAsync(s->prev_plane, ctx->ws.oplane, P * sizeof(float4), hipMemcpyDeviceToDevice,
                                 ctx->stream));
