@@ -50,12 +50,43 @@ __global__ void k_fill_u32(unsigned int* __restrict__ p, int n, unsigned int v) 
     if (k < n) p[k] = v;
 }
 
+// An init tile is 4 rows x 64 columns, one row per wave.  Its reference
+// patch and bearing-table entries (the tile plus a halo of half each side,
+// coordinates clamped: only interior pixels, whose taps lie inside the image,
+// read them) are staged in LDS, as for a propagation tile, so that the tap
+// reads never wait behind the neighbour gathers issued before them.
+constexpr int kInitRows = 4;
+constexpr int kInitStageRows = kInitRows + 2 * kMaxHalf;
+struct InitStage {
+    float2 row[kInitStageRows];
+    float2 col[kTileCols];
+    uint8_t ref[kInitStageRows * kTileCols];
+};
+static_assert(64 + 2 * kMaxHalf <= kTileCols, "init tile columns");
+
+// Waits for the previous tile's readers first.
+__device__ __forceinline__ LdsTaps stage_init_tile(const KfArgs& a, InitStage& st, int tx, int ty) {
+    const int r0 = ty * kInitRows - a.half, c0 = 64 * tx - a.half;
+    const int nr = kInitRows + 2 * a.half, nc = 64 + 2 * a.half;
+    __syncthreads();
+    for (int e = threadIdx.x; e < nc; e += 256) st.col[e] = a.coltab[min(max(c0 + e, 0), a.W - 1)];
+    for (int e = threadIdx.x; e < nr; e += 256) st.row[e] = a.rowtab[min(max(r0 + e, 0), a.H - 1)];
+    for (int e = threadIdx.x; e < nr * nc; e += 256) {
+        const int r = e / nc, c = e - r * nc;
+        const int y = min(max(r0 + r, 0), a.H - 1), x = min(max(c0 + c, 0), a.W - 1);
+        st.ref[r * kTileCols + c] = a.ref[(size_t)y * a.W + x];
+    }
+    __syncthreads();
+    return LdsTaps{st.row, st.col, st.ref, r0, c0};
+}
+
 // kScore: only score the working planes (pmp_score_plane_depth); otherwise
 // random initialisation + score + snapshot.
 template <bool kZero, bool kScore>
 __global__ __launch_bounds__(256) void k_init_q(const SlotDesc PMP_CONST* slots, const BatchHdr PMP_CONST* hdr) {
     __shared__ float sWc[256];
     __shared__ float sv[kMaxViews][256];
+    __shared__ InitStage st;
     const WorkList PMP_CONST* L = &hdr->init;
     const int total = L->total;
     int cur = -1;
@@ -68,17 +99,19 @@ __global__ __launch_bounds__(256) void k_init_q(const SlotDesc PMP_CONST* slots,
             load_wc(d.a.Wc, sWc);
             cur = s;
         }
-        const int j = (local % d.nbx_init) * 64 + (threadIdx.x & 63);
-        const int i = (local / d.nbx_init) * 4 + (threadIdx.x >> 6);
+        const int tx = local % d.nbx_init, ty = local / d.nbx_init;
+        const LdsTaps tp = stage_init_tile(d.a, st, tx, ty);
+        const int j = tx * 64 + (threadIdx.x & 63);
+        const int i = ty * kInitRows + (threadIdx.x >> 6);
         if (i >= d.a.H || j >= d.a.W) continue;
         if constexpr (kScore) {
             if (!in_score_grid(d.a, i, j)) continue;
             const size_t k = (size_t)i * d.a.W + j;
             const RefStat rs = ref_stats(d.a, sWc, i, j);
             const float4 p = d.plane[k];
-            d.score[k] = score_plane<0, kZero>(d.a, sWc, rs, i, j, p.x, p.y, p.z, &sv[0][threadIdx.x], 256);
+            d.score[k] = score_plane_t<0, kZero>(d.a, tp, sWc, rs, i, j, p.x, p.y, p.z, &sv[0][threadIdx.x], 256);
         } else {
-            init_pixel<0, kZero>(d.a, sWc, i, j, d.plane, d.depth, d.score, d.iplane, d.idepth, d.iscore,
+            init_pixel<0, kZero>(d.a, tp, sWc, i, j, d.plane, d.depth, d.score, d.iplane, d.idepth, d.iscore,
                                  &sv[0][threadIdx.x], 256);
         }
     }

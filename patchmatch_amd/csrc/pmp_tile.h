@@ -32,18 +32,20 @@ __device__ __forceinline__ int xcd_tile(int b, int nb8, int nb) {
 
 // Random hypothesis, initial score and snapshot of pixel (i, j)
 // (RandomInitialization, PatchMatchCuda.cu:210-270, depthmap.cc:656-663).
-// sv/ld: per-view score scratch of the calling thread (NV == 0 only).
-template <int NV, bool kZero>
-__device__ __forceinline__ void init_pixel(const KfArgs& a, const float* sWc, int i, int j, float4* __restrict__ plane,
-                                           float* __restrict__ depth, float* __restrict__ score,
-                                           float4* __restrict__ iplane, float* __restrict__ idepth,
-                                           float* __restrict__ iscore, float* sv = nullptr, int ld = 0) {
+// sv/ld: per-view score scratch of the calling thread (NV == 0 only); tp:
+// where the reference taps come from (GlobalTaps or a staged LdsTaps).
+template <int NV, bool kZero, class Taps>
+__device__ __forceinline__ void init_pixel(const KfArgs& a, const Taps& tp, const float* sWc, int i, int j,
+                                           float4* __restrict__ plane, float* __restrict__ depth,
+                                           float* __restrict__ score, float4* __restrict__ iplane,
+                                           float* __restrict__ idepth, float* __restrict__ iscore,
+                                           float* sv = nullptr, int ld = 0) {
     float p0 = 0.0f, p1 = 0.0f, p2 = 0.0f, dep = 0.0f;
     if (in_init_grid(a, i, j)) random_plane(a, i, j, p0, p1, p2, dep);
     float sc = __uint_as_float(0xFFFFFFFFu);   // cudaMemset(score, 0xFF) (depthmap.cc:657)
     if (in_score_grid(a, i, j)) {
         RefStat rs = ref_stats(a, sWc, i, j);
-        sc = score_plane<NV, kZero>(a, sWc, rs, i, j, p0, p1, p2, sv, ld);
+        sc = score_plane_t<NV, kZero>(a, tp, sWc, rs, i, j, p0, p1, p2, sv, ld);
     }
     const size_t k = (size_t)i * a.W + j;
     const float4 pl = make_float4(p0, p1, p2, 0.0f);
