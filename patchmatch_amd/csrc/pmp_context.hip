@@ -138,6 +138,7 @@ int ensure_workspace(pmp_ctx* ctx, int W, int H) {
         PMP_HIP(ctx, hipStreamSynchronize(ctx->stream));
         if (ctx->mask) (void)hipFree(ctx->mask);
         ctx->mask = nullptr;
+        ctx->mask_held = 0;
         int rc = ensure_slot(ctx, ctx->ws, W, H);
         if (rc) return rc;
         if ((rc = alloc_n(ctx, ctx->mask, P))) return rc;
@@ -281,6 +282,8 @@ void pmp_destroy(pmp_ctx* ctx) {
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     if (ctx->stage) (void)hipHostFree(ctx->stage);
+    if (ctx->mask_pin) (void)hipHostFree(ctx->mask_pin);
+    if (ctx->mask_copied) (void)hipEventDestroy(ctx->mask_copied);
     for (auto& kv : ctx->rowtabs) (void)hipFree(kv.second);
     for (auto& kv : ctx->coltabs) (void)hipFree(kv.second);
     for (auto& kv : ctx->wcs) (void)hipFree(kv.second);

@@ -151,11 +151,37 @@ int plain_prepare(pmp_ctx* ctx, const pmp_params* p, const uint8_t* dmask, bool 
 }
 
 // A host mask goes to the workspace's staging buffer first.
+// A host mask goes through a pinned copy, so the upload is a plain DMA that
+// does not hold the host until the stream drains; a mask with the same bytes
+// as the last one uploaded (a camera's fixed mask, or the same keyframe's mask
+// in init, warm start and finalize) is not uploaded again: the device copy
+// still holds it in stream order.
 int stage_mask(pmp_ctx* ctx, const uint8_t* mask_host, const uint8_t*& dmask) {
     dmask = nullptr;
     if (!mask_host) return PMP_OK;
     const size_t P = (size_t)ctx->W * ctx->H;
-    PMP_HIP(ctx, hipMemcpyAsync(ctx->mask, mask_host, P, hipMemcpyHostToDevice, ctx->stream));
+    if (ctx->mask_held == P && memcmp(ctx->mask_pin, mask_host, P) == 0) {
+        dmask = ctx->mask;
+        return PMP_OK;
+    }
+    ctx->mask_held = 0;
+    if (!ctx->mask_copied) PMP_HIP(ctx, hipEventCreateWithFlags(&ctx->mask_copied, hipEventDisableTiming));
+    if (ctx->mask_pending) {
+        PMP_HIP(ctx, hipEventSynchronize(ctx->mask_copied));   // the last upload out of mask_pin is done
+        ctx->mask_pending = false;
+    }
+    if (P > ctx->mask_pin_cap) {
+        if (ctx->mask_pin) (void)hipHostFree(ctx->mask_pin);
+        ctx->mask_pin = nullptr;
+        ctx->mask_pin_cap = 0;
+        PMP_HIP(ctx, hipHostMalloc((void**)&ctx->mask_pin, P, hipHostMallocDefault));
+        ctx->mask_pin_cap = P;
+    }
+    memcpy(ctx->mask_pin, mask_host, P);
+    PMP_HIP(ctx, hipMemcpyAsync(ctx->mask, ctx->mask_pin, P, hipMemcpyHostToDevice, ctx->stream));
+    PMP_HIP(ctx, hipEventRecord(ctx->mask_copied, ctx->stream));
+    ctx->mask_pending = true;
+    ctx->mask_held = P;
     dmask = ctx->mask;
     return PMP_OK;
 }

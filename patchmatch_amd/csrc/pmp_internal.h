@@ -105,6 +105,11 @@ struct pmp_ctx {
     pmp::SlotBuf ws;
     uint64_t pool_clock = 0;          // advances once per request that reads the pool
     uint8_t* mask = nullptr;          // staging of a host mask (ws.cap bytes)
+    uint8_t* mask_pin = nullptr;      // pinned copy of the last host mask uploaded into mask
+    size_t mask_pin_cap = 0;
+    size_t mask_held = 0;             // bytes of that mask that mask holds (0: none)
+    hipEvent_t mask_copied = nullptr; // recorded after the upload out of mask_pin
+    bool mask_pending = false;
     std::unordered_map<int, float2*> rowtabs;   // by H: (cos lat, sin lat)
     std::unordered_map<int, float2*> coltabs;   // by W: (sin lon, cos lon)
     std::unordered_map<uint32_t, float*> wcs;   // by the bits of sigma_color: exp(-a^2 / (2 sigma^2)), a < 256

@@ -452,3 +452,33 @@ def test_golden_fixture_on_gpu(gpu_ctx):
     assert _same_bits(gp[..., :3], fx["plane"]).all() and np.array_equal(gn, fx["nghbr"])
     wp, wd, ws = gpu_ctx.get_maps(0)
     assert _same_bits(wd, fx["raw_depth"]).all() and _same_bits(wp, fx["raw_plane"]).all()
+
+
+def test_host_mask_changes_between_calls(gpu_ctx, kf_c0):
+    """A host mask is uploaded through a pinned copy and skipped when its bytes
+    equal the last upload's: finalize with mask A, then B (same size, other
+    bytes), then A again must give what the pooled masks A and B give."""
+    p = config0_params()
+    _load(gpu_ctx, kf_c0)
+    a = kf_c0.mask.copy()
+    b = a.copy()
+    b[:, : b.shape[1] // 2] = 0
+    assert not np.array_equal(a, b)
+    gpu_ctx.random_initialization(p)
+    gpu_ctx.red_black_pass(p, 0, 1)
+    got = []
+    for m in (a, b, a):
+        gpu_ctx.finalize(p, m)
+        got.append(gpu_ctx.get_maps(2)[1].copy())
+    want = []
+    for k, m in enumerate((a, b)):
+        gpu_ctx.image_put(9100 + k, m)
+        gpu_ctx.set_mask(9100 + k)
+        gpu_ctx.finalize(p)
+        want.append(gpu_ctx.get_maps(2)[1].copy())
+    gpu_ctx.set_mask(-1)
+    for k in range(2):
+        gpu_ctx.image_drop(9100 + k)
+    assert not np.array_equal(want[0], want[1])
+    assert np.array_equal(got[0], want[0]) and np.array_equal(got[2], want[0])
+    assert np.array_equal(got[1], want[1])
