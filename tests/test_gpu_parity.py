@@ -117,6 +117,33 @@ def test_score_field_bitwise(gpu_ctx, kf_small6, agg):
     assert _report("score", gs[ok], os_[ok]).all()
 
 
+@pytest.mark.parametrize("edge", [EDGE_CLAMP, EDGE_ZERO])
+def test_init_and_score_ragged_tiles(gpu_ctx, edge):
+    """Init and score kernels stage 4 x 64 tiles of the reference with a halo
+    of the patch half-width: at 136 x 70 (a partial tile column of 8 pixels,
+    a partial tile row of 2) with the largest half-width (11 x 11 at stride
+    2), every map bit for bit against the oracle."""
+    from patchmatch_amd import synth
+    kf = cached("ragged", lambda: synth.config1(4, W=136, H=70, n_views=3, base_freq=synth.TEX_FREQ_C0))
+    p = config1_params(coverage_mode=COVER_FULL, edge_mode=edge, seed=11)
+    assert p.patch_half * p.patch_stride == 10
+    _load(gpu_ctx, kf)
+    print(f"case ragged edge {edge}", flush=True)
+    gpu_ctx.random_initialization(p)
+    gp, gd, gs = gpu_ctx.get_maps(0)
+    op, od, os_ = or_py.init(or_py.Keyframe.from_synth(kf), p)
+    assert _report("plane", gp, op).all()
+    assert _report("depth", gd, od).all()
+    assert _report("score", gs, os_).all()
+    planes = random_planes(kf, 9)
+    gpu_ctx.set_maps(planes, np.ones((kf.H, kf.W), np.float32), np.zeros((kf.H, kf.W), np.float32))
+    gpu_ctx.score_plane_depth(p)
+    _, _, gs = gpu_ctx.get_maps(0)
+    os_ = or_py.score_field(or_py.Keyframe.from_synth(kf), p, planes)
+    ok = ~np.isnan(os_)
+    assert _report("score field", gs[ok], os_[ok]).all()
+
+
 def test_propagate_c0(gpu_ctx, kf_c0):
     """Config 0 (480x240, 2 views, 7x7 patch, the reference's pooled fold,
     reference launch grids): init + 4 red-black iterations, every map bit for
