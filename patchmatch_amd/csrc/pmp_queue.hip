@@ -2,7 +2,8 @@
 //
 // Every stage is one launch of a persistent kernel whose blocks walk a list
 // of work items over all slots of a descriptor set:
-//   k_init_q       random init + initial score; items (slot, 64x4 tile)
+//   k_init_q       random init + initial score; items (slot, 64x4 tile), the
+//                  tile's reference taps staged in LDS
 //   k_score_q      score of the working planes; items (slot, 64x4 tile)
 //   k_propagate_q  one red-black pass; items (slot, R x 64 same-colour tile),
 //                  split into 8 contiguous bands, one per XCD, each taken
